@@ -1,0 +1,107 @@
+/* C ABI of liblippmannschwinger.so, the MI355X (gfx950) Lippmann-Schwinger
+ * solver.  All functions are extern "C", take plain pointers and sizes, and
+ * run synchronously on the calling thread's current HIP device.
+ *
+ * Conventions (same as the reference solver):
+ *   - scalar fields are float32 [nx][ny][nz], C order (z fastest);
+ *   - strain / stress fields are float32 [6][nx][ny][nz], Voigt order
+ *     (xx, yy, zz, xy, xz, yz) with tensor (not engineering) shear components;
+ *   - voxels = {nx, ny, nz} (int32), extents = {Lx, Ly, Lz} (float32).
+ */
+#ifndef LIPPMANN_SCHWINGER_H
+#define LIPPMANN_SCHWINGER_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Drop-in replacement of the reference entry point
+ * (cuda/src/lippmann_schwinger.hh:176-184, implemented at
+ * cuda/src/lippmann_schwinger.cu:236-258; bound by ctypes in
+ * src/jaxmaterials/solver/lippmann_schwinger.py:268-329).
+ *
+ * Host pointers.  Solves for the strain with mean epsilon_bar[6] in the medium
+ * with Lame fields lambda[N], mu[N]; writes epsilon and sigma ([6][N] each).
+ * Stops when the relative residual R < max(rtol * R0, atol), R0 = residual of
+ * the uniform start.  Returns the number of strain updates; maxiter means not
+ * converged (sigma is then the stress before the last update, as in the
+ * reference).  Differences: a HIP error or invalid argument prints a message
+ * to stderr and returns -1 (the reference prints and carries on); a
+ * non-finite residual stops the loop and returns maxiter. */
+int lippmann_schwinger_solve(float *lambda, float *mu, float *epsilon_bar,
+                             float *epsilon, float *sigma, int *voxels,
+                             float *extents, float rtol, float atol,
+                             int maxiter, int verbose);
+
+/* ---- additive entry points (no counterpart in the reference) ---------- */
+
+/* Effective 6x6 tensor of one microstructure (host pointers).  Row k is the
+ * volume average of sigma under epsilon_bar = e_k (the C_eff loop of
+ * src/jaxmaterials/data/data.py:178-186), all six load cases batched in one
+ * solve; averages in FP64.  depth = Anderson depth (0 = plain iteration).
+ * iters[6] receives the iterations per load case.  Returns 0 on success,
+ * 1 if a load case did not converge, -1 on error. */
+int lippmann_schwinger_effective_tensor(const float *lambda, const float *mu,
+                                        const int *voxels, const float *extents,
+                                        float rtol, float atol, int maxiter,
+                                        int depth, int verbose, double *ceff,
+                                        int *iters);
+
+/* Reusable solver on device memory (used by the Python/PyTorch layer).
+ * nrhs load cases share nmat Lame-field pairs; depth = Anderson depth (<= 8).
+ * stream = hipStream_t to run on (NULL: the solver creates its own). */
+typedef struct ls_solver ls_solver;
+
+ls_solver *ls_solver_create(const int *voxels, const float *extents, int nrhs,
+                            int nmat, int depth, void *stream);
+
+/* Solve all load cases.  Device pointers: d_lambda/d_mu [nmat][N];
+ * d_tau [nrhs][6][N] or NULL (polarisation added to the stress);
+ * d_eps [nrhs][6][N] in/out (input = warm start when warm_start != 0, it is
+ * projected onto compatible fields with mean eps_bar and replaced by the
+ * uniform start if its residual exceeds the uniform start's); d_sigma
+ * [nrhs][6][N] or NULL.  Host arrays: rhs_mat[nrhs] (material of each load
+ * case), eps_bar[nrhs][6]; outputs iters[nrhs], residual[nrhs] (final
+ * relative residual), sigma_mean[nrhs][6] (FP64 volume averages), each may
+ * be NULL.  The relative tolerance is measured against the residual of the
+ * uniform start R(eps_bar) in every case; rel_ref > 0 overrides it (used by
+ * the adjoint solve).  Returns 0 if all converged, 1 if some did not, -1 on
+ * error. */
+int ls_solver_solve(ls_solver *s, const float *d_lambda, const float *d_mu,
+                    const int *rhs_mat, const float *eps_bar,
+                    const float *d_tau, float *d_eps, float *d_sigma,
+                    int warm_start, float rtol, float atol, int maxiter,
+                    double rel_ref, int verbose, int *iters, double *residual,
+                    double *sigma_mean);
+
+/* Kernel time of the last solve, ms per pass kind: [0] z-line passes,
+ * [1] y passes, [2] x passes, [3] everything else; and pass counts. */
+int ls_solver_last_timing(ls_solver *s, double *ms4, int *count4);
+
+void ls_solver_destroy(ls_solver *s);
+
+/* Seeded cross-ply fibre microstructure (random sequential adsorption) and
+ * its rasterisation into Lame fields on the GPU.
+ *   nlayers layers along z; layer k has fibres along x (k odd) or y (k even);
+ *   fibre radius, target volume fraction vf per layer; seed of the
+ *   splitmix64 stream.  Writes d_lambda, d_mu [N] (device pointers) and
+ *   returns the number of fibres placed (or -1 on error); achieved_vf
+ *   receives the rasterised fibre volume fraction. */
+int ls_crossply_lame(const int *voxels, const float *extents, int nlayers,
+                     double radius, double vf, unsigned long long seed,
+                     double lam_matrix, double mu_matrix, double lam_fibre,
+                     double mu_fibre, float *d_lambda, float *d_mu,
+                     double *achieved_vf);
+
+/* Host-only RSA placement (same stream and rules as ls_crossply_lame).
+ * Writes up to cap fibres as (layer, in-plane coordinate, z) triples into
+ * out[cap][3]; returns the count. */
+int ls_crossply_place(const int *voxels, const float *extents, int nlayers,
+                      double radius, double vf, unsigned long long seed,
+                      double *out, int cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LIPPMANN_SCHWINGER_H */

@@ -1,0 +1,81 @@
+"""ctypes binding of liblippmannschwinger.so (declarations of include/lippmann_schwinger.h)."""
+
+import ctypes
+import os
+
+from .build import LIB_PATH, LIB_NAME
+
+_LIB = None
+
+c_float_p = ctypes.POINTER(ctypes.c_float)
+c_int_p = ctypes.POINTER(ctypes.c_int)
+c_double_p = ctypes.POINTER(ctypes.c_double)
+
+# name -> (restype, argtypes); the full exported surface of the C ABI
+SIGNATURES = {
+    "lippmann_schwinger_solve": (
+        ctypes.c_int,
+        [ctypes.c_void_p] * 7 + [ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_int],
+    ),
+    "lippmann_schwinger_effective_tensor": (
+        ctypes.c_int,
+        [ctypes.c_void_p] * 4 + [ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_void_p, ctypes.c_void_p],
+    ),
+    "ls_solver_create": (
+        ctypes.c_void_p,
+        [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p],
+    ),
+    "ls_solver_solve": (
+        ctypes.c_int,
+        [ctypes.c_void_p] * 8 + [ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_double,
+                                 ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+    ),
+    "ls_solver_last_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "ls_solver_destroy": (None, [ctypes.c_void_p]),
+    "ls_crossply_lame": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_ulonglong,
+         ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p,
+         ctypes.c_void_p],
+    ),
+    "ls_crossply_place": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_ulonglong,
+         ctypes.c_void_p, ctypes.c_int],
+    ),
+}
+
+
+def library_path():
+    return LIB_PATH
+
+
+def load():
+    """Load the in-tree library (or one named liblippmannschwinger.so on the
+    loader path); raise RuntimeError if neither loads, like the reference
+    loader (src/jaxmaterials/solver/lippmann_schwinger.py:285-290)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    errors = []
+    for cand in (LIB_PATH, LIB_NAME):
+        if cand == LIB_PATH and not os.path.exists(cand):
+            errors.append(f"{cand}: not built")
+            continue
+        try:
+            lib = ctypes.CDLL(cand)
+            break
+        except OSError as exc:
+            errors.append(f"{cand}: {exc}")
+    else:
+        raise RuntimeError(
+            "Unable to load HIP library liblippmannschwinger.so. Build it with "
+            "`python -m jaxmaterials_amd.build` or check LD_LIBRARY_PATH. (" + "; ".join(errors) + ")"
+        )
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = lib
+    return lib

@@ -1,0 +1,654 @@
+// HIP kernels of the Lippmann-Schwinger iteration for gfx950.
+//
+// Data layout in HBM (one solver instance, R right-hand sides = load cases):
+//   lambda, mu : [nmat][N] fp32, C order (x slowest, z fastest)
+//   eps        : [R][6][N] fp32, Voigt (xx,yy,zz,xy,xz,yz), tensor shear
+//   tau        : [R][6][N] fp32, optional polarisation added to the stress
+//   spec       : [R][nx][ny][P][6] complex fp32, P = nz/2+1 (z half spectrum),
+//                the six Voigt components of one mode adjacent (48 B)
+// One iteration of the plain scheme is four passes over the spectrum:
+//   k_zline   : per (x,y) line: C2R along z -> eps += r -> Hooke -> R2C along z
+//   k_ypass   : forward FFT along y
+//   k_xpass   : forward FFT along x -> residual + Gamma0 -> inverse FFT along x
+//   k_ypass   : inverse FFT along y
+// followed by k_check (deterministic FP64 residual reduction, stop test).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fft_lds.hpp"
+
+namespace lsk {
+
+constexpr int kMaxDepth1 = 9;  // Anderson depth <= 8
+
+struct GridDev {
+  int nx, ny, nz, P;
+  long long N;  // nx*ny*nz
+  long long M;  // nx*ny*P
+  float gx, gy, gz;  // 2 n_d / L_d
+  const float* sx;   // sin(pi k / n) per axis, exact 0 at k = 0
+  const float* cx;   // cos(pi k / n) per axis, exact 0 at k = n/2
+  const float* sy;
+  const float* cy;
+  const float* sz;
+  const float* cz;
+  AxisPlan px, py, pz;
+};
+
+struct RhsDesc {
+  int mat;          // index of the (lambda, mu) pair
+  float eps_bar[6];
+  float cA, cB;     // Gamma0 constants, 1/N of the unnormalised inverse folded in
+  float lam0, mu0;  // reference medium
+};
+
+struct RhsState {
+  double R0;    // normaliser of the relative tolerance
+  double R;     // latest residual
+  double den;   // ||sigma_hat(0)||_V of the latest check
+  int checks;   // residual checks done
+  int active;   // 1 while the load case iterates
+  int converged;
+  int status;   // 0 ok, 1 non-finite residual
+};
+
+// z-line pass modes
+enum ZMode : int {
+  Z_INIT = 0,  // eps = eps_bar (written), sigma = C eps (+tau), R2C
+  Z_FWD = 1,   // eps read, sigma = C eps (+tau), R2C
+  Z_UPD = 2,   // C2R -> eps += r -> sigma -> R2C
+  Z_SET = 3,   // C2R -> eps = eps_bar - r -> sigma -> R2C   (end of a projection)
+  Z_PROJ = 4,  // eps read, sigma = C0 (eps - eps_bar), R2C  (start of a projection)
+  Z_AAINV = 5, // C2R -> r; history slot <- r, g = eps + r; FP64 Gram-row partials (no R2C)
+  Z_AAMIX = 6, // eps = sum_s alpha_s g^(s) -> sigma -> R2C
+};
+
+// Anderson history (lippmann_schwinger.py:177-231, doc/linear_elasticity.tex:335-370),
+// stored per load case as ring buffers of g^(s) = eps^(s) + r^(s) and r^(s).
+struct AAParams {
+  float* G;              // [R][m1][6][N]
+  float* Rh;             // [R][m1][6][N]
+  const double* alpha;   // [R][m1] mixing weights by slot
+  double* dots;          // [R][nblk][m1] Gram-row partials by age
+  const int* head;       // [R] slot of the newest entry for this step
+  const int* filled;     // [R] number of filled slots including the newest
+  int m1;                // depth + 1
+};
+
+__device__ __forceinline__ float4 ld4(const float2* p) { return *reinterpret_cast<const float4*>(p); }
+__device__ __forceinline__ void st4(float2* p, float4 v) { *reinterpret_cast<float4*>(p) = v; }
+
+// Hooke's law, isotropic: sigma_i = 2 mu eps_i + lambda tr(eps) [i<3]
+// (src/jaxmaterials/solver/lippmann_schwinger.py:20-33).
+__device__ __forceinline__ void hooke(const float* e, float lam, float mu, float* s) {
+  const float tr = e[0] + e[1] + e[2];
+  const float m2 = 2.f * mu;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) s[c] = fmaf(m2, e[c], lam * tr);
+#pragma unroll
+  for (int c = 3; c < 6; ++c) s[c] = m2 * e[c];
+}
+
+// ---------------------------------------------------------------------------
+// wave64 / block reductions in FP64 with a fixed order (deterministic)
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < kThreads / 64; ++i) s += red[i];
+  }
+  return s;  // valid in thread 0
+}
+
+// ---------------------------------------------------------------------------
+// z-line pass.  Block = Lz consecutive (x,y) lines of one load case.
+// LDS: Z[l][p][k], p = component pair (0,1),(2,3),(4,5): z_p = s_{2p} + i s_{2p+1}.
+template <int MODE, int E>
+__global__ void __launch_bounds__(kThreads) k_zline(GridDev g, const RhsDesc* __restrict__ rhs,
+                                                    RhsState* __restrict__ st, const float* __restrict__ lam,
+                                                    const float* __restrict__ mu, float* __restrict__ eps,
+                                                    const float* __restrict__ tau, float2* __restrict__ spec,
+                                                    int Lz, AAParams aa, int write_eps) {
+  extern __shared__ float2 lds[];
+  __shared__ double red[kThreads / 64];
+  const int r = blockIdx.y;
+  if (!st[r].active) return;
+  const int nz = g.nz, P = g.P;
+  const int nlines_xy = g.nx * g.ny;
+  const int line0 = blockIdx.x * Lz;
+  const int L = min(Lz, nlines_xy - line0);
+  const RhsDesc d = rhs[r];
+  const long long N = g.N;
+  float2* S = spec + (long long)r * g.M * 6;
+  float* E_ = eps + (long long)r * 6 * N;
+  const float* T_ = tau ? tau + (long long)r * 6 * N : nullptr;
+  const float* LA = lam + (long long)d.mat * N;
+  const float* MU = mu + (long long)d.mat * N;
+  const int ls = nz;
+  const int tid = threadIdx.x;
+
+  if constexpr (MODE == Z_UPD || MODE == Z_SET || MODE == Z_AAINV) {
+    // gather the half spectra into the paired full-length lines (C2R semantics:
+    // imaginary parts of the k = 0 and k = nz/2 coefficients are ignored)
+    const int items = L * P * 3;
+    for (int e = tid; e < items; e += kThreads) {
+      const int p = e % 3;
+      const int k = (e / 3) % P;
+      const int l = e / (3 * P);
+      float4 ab = ld4(S + ((long long)(line0 + l) * P + k) * 6 + 2 * p);
+      float2 A = make_float2(ab.x, ab.y), B = make_float2(ab.z, ab.w);
+      const bool self = (k == 0) || (2 * k == nz);
+      if (self) { A.y = 0.f; B.y = 0.f; }
+      float2* Z = lds + (l * 3 + p) * ls;
+      Z[k] = make_float2(A.x - B.y, A.y + B.x);
+      if (!self) Z[nz - k] = make_float2(A.x + B.y, B.x - A.y);
+    }
+    __syncthreads();
+    fft_lines<true, false, E>(lds, g.pz, 3 * L, 1, ls);
+  }
+
+  if constexpr (MODE == Z_AAINV) {
+    // store r and g = eps + r in the newest slot; Gram-row partials by age
+    const int m1 = aa.m1;
+    const int h = aa.head[r];
+    const int f = aa.filled[r];
+    const long long slot_stride = 6 * N;
+    float* Gb = aa.G + (long long)r * m1 * slot_stride;
+    float* Rb = aa.Rh + (long long)r * m1 * slot_stride;
+    double acc[kMaxDepth1];
+#pragma unroll
+    for (int s = 0; s < kMaxDepth1; ++s) acc[s] = 0.0;
+    const int items = L * nz;
+    for (int e = tid; e < items; e += kThreads) {
+      const int l = e / nz, j = e - l * nz;
+      const long long v = (long long)(line0 + l) * nz + j;
+      float rr[6];
+#pragma unroll
+      for (int p = 0; p < 3; ++p) {
+        const float2 z = lds[(l * 3 + p) * ls + j];
+        rr[2 * p] = z.x;
+        rr[2 * p + 1] = z.y;
+      }
+      float* Rn = Rb + h * slot_stride;
+      float* Gn = Gb + h * slot_stride;
+      double a0 = 0.0;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        const float ec = E_[c * N + v];
+        Rn[c * N + v] = rr[c];
+        Gn[c * N + v] = ec + rr[c];
+        a0 += (c < 3 ? 1.0 : 2.0) * (double)rr[c] * rr[c];
+      }
+      acc[0] += a0;
+#pragma unroll
+      for (int s = 1; s < kMaxDepth1; ++s) {
+        if (s < f) {
+          int sl = h - s;
+          if (sl < 0) sl += m1;
+          const float* Rs = Rb + sl * slot_stride;
+          double a = 0.0;
+#pragma unroll
+          for (int c = 0; c < 6; ++c) a += (c < 3 ? 1.0 : 2.0) * (double)rr[c] * Rs[c * N + v];
+          acc[s] += a;
+        }
+      }
+    }
+    for (int s = 0; s < m1; ++s) {
+      const double bs = block_sum(acc[s], red);
+      if (tid == 0) aa.dots[((long long)r * gridDim.x + blockIdx.x) * m1 + s] = (s < f) ? bs : 0.0;
+    }
+    return;
+  }
+
+  // pointwise: update / Hooke, write sigma pairs into LDS
+  {
+    const int items = L * nz;
+    for (int e = tid; e < items; e += kThreads) {
+      const int l = e / nz, j = e - l * nz;
+      const long long v = (long long)(line0 + l) * nz + j;
+      float ev[6];
+      if constexpr (MODE == Z_INIT) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+          ev[c] = d.eps_bar[c];
+          if (write_eps) E_[c * N + v] = ev[c];
+        }
+      } else if constexpr (MODE == Z_FWD || MODE == Z_PROJ) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) ev[c] = E_[c * N + v];
+      } else if constexpr (MODE == Z_AAMIX) {
+        const int m1 = aa.m1;
+        const int f = aa.filled[r];
+        const int h = aa.head[r];
+        const long long slot_stride = 6 * N;
+        const float* Gb = aa.G + (long long)r * m1 * slot_stride;
+        const double* al = aa.alpha + (long long)r * m1;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) ev[c] = 0.f;
+        for (int s = 0; s < f; ++s) {
+          int sl = h - s;
+          if (sl < 0) sl += m1;
+          const float a = (float)al[sl];
+          const float* Gs = Gb + sl * slot_stride;
+#pragma unroll
+          for (int c = 0; c < 6; ++c) ev[c] = fmaf(a, Gs[c * N + v], ev[c]);
+        }
+#pragma unroll
+        for (int c = 0; c < 6; ++c) E_[c * N + v] = ev[c];
+      } else {
+        float rr[6];
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+          const float2 z = lds[(l * 3 + p) * ls + j];
+          rr[2 * p] = z.x;
+          rr[2 * p + 1] = z.y;
+        }
+        if constexpr (MODE == Z_UPD) {
+#pragma unroll
+          for (int c = 0; c < 6; ++c) ev[c] = E_[c * N + v] + rr[c];
+        } else {
+#pragma unroll
+          for (int c = 0; c < 6; ++c) ev[c] = d.eps_bar[c] - rr[c];
+        }
+#pragma unroll
+        for (int c = 0; c < 6; ++c) E_[c * N + v] = ev[c];
+      }
+      float s[6];
+      if constexpr (MODE == Z_PROJ) {
+        float de[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) de[c] = ev[c] - d.eps_bar[c];
+        hooke(de, d.lam0, d.mu0, s);
+      } else {
+        hooke(ev, LA[v], MU[v], s);
+        if (T_) {
+#pragma unroll
+          for (int c = 0; c < 6; ++c) s[c] += T_[c * N + v];
+        }
+      }
+#pragma unroll
+      for (int p = 0; p < 3; ++p) lds[(l * 3 + p) * ls + j] = make_float2(s[2 * p], s[2 * p + 1]);
+    }
+  }
+  __syncthreads();
+  fft_lines<false, false, E>(lds, g.pz, 3 * L, 1, ls);
+
+  // split the paired transforms into two half spectra each and store
+  {
+    const int items = L * P * 3;
+    for (int e = tid; e < items; e += kThreads) {
+      const int p = e % 3;
+      const int k = (e / 3) % P;
+      const int l = e / (3 * P);
+      const float2* Z = lds + (l * 3 + p) * ls;
+      const float2 zk = Z[k];
+      const float2 zm = Z[k == 0 ? 0 : nz - k];
+      float4 ab;
+      ab.x = 0.5f * (zk.x + zm.x);
+      ab.y = 0.5f * (zk.y - zm.y);
+      ab.z = 0.5f * (zk.y + zm.y);
+      ab.w = -0.5f * (zk.x - zm.x);
+      st4(S + ((long long)(line0 + l) * P + k) * 6 + 2 * p, ab);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// y pass: FFT along y of T consecutive (kz, component) columns at fixed x.
+template <bool INV, int E>
+__global__ void __launch_bounds__(kThreads) k_ypass(GridDev g, const RhsState* __restrict__ st,
+                                                    float2* __restrict__ spec, int T, int ntiles) {
+  extern __shared__ float2 lds[];
+  const int r = blockIdx.y;
+  if (!st[r].active) return;
+  const int ny = g.ny;
+  const int Q = 6 * g.P;  // columns per (x, y)
+  const int x = blockIdx.x / ntiles;
+  const int q0 = (blockIdx.x - x * ntiles) * T;
+  float2* S = spec + (long long)r * g.M * 6 + (long long)x * ny * Q;
+  const int tid = threadIdx.x;
+  const int items = ny * T;
+  for (int e = tid; e < items; e += kThreads) {
+    const int y = e / T, t = e - y * T;
+    const int q = q0 + t;
+    lds[e] = (q < Q) ? S[(long long)y * Q + q] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  fft_lines<INV, true, E>(lds, g.py, T, T, 1);
+  for (int e = tid; e < items; e += kThreads) {
+    const int y = e / T, t = e - y * T;
+    const int q = q0 + t;
+    if (q < Q) S[(long long)y * Q + q] = lds[e];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// x pass: forward FFT along x -> residual (FP64 partial per block), DC capture,
+// Gamma0 -> inverse FFT along x.  Block = Tm consecutive (ky,kz) modes of one
+// load case, all six components, full x extent.
+// Residual (lippmann_schwinger.py:118-143, fourier.cu:140-265):
+//   sum_k w_kz |xi~(k) . sigma_hat(k)|^2, w = 1 for kz = 0 or kz = nz/2, else 2.
+// Gamma0 (fourier.py:115-186, doc/linear_elasticity.tex:137-226), with n = xi0:
+//   v = sigma_hat n,  rho = n . v,
+//   r_hat = cA (n0 v0, n1 v1, n2 v2, (n0 v1 + n1 v0)/2, (n0 v2 + n2 v0)/2,
+//               (n1 v2 + n2 v1)/2) + cB rho (n0^2, n1^2, n2^2, n0 n1, n0 n2, n1 n2)
+template <int E>
+__global__ void __launch_bounds__(kThreads) k_xpass(GridDev g, const RhsDesc* __restrict__ rhs,
+                                                    const RhsState* __restrict__ st, float2* __restrict__ spec,
+                                                    double* __restrict__ partials, float* __restrict__ dc, int Tm,
+                                                    int do_gamma) {
+  extern __shared__ float2 lds[];
+  __shared__ double red[kThreads / 64];
+  const int r = blockIdx.y;
+  if (!st[r].active) return;
+  const int nx = g.nx, P = g.P;
+  const long long nm = (long long)g.ny * P;  // modes per x plane
+  const int m0 = blockIdx.x * Tm;
+  const int cols = 6 * Tm;
+  float2* S = spec + (long long)r * g.M * 6;
+  const RhsDesc d = rhs[r];
+  const int tid = threadIdx.x;
+  const long long colmax = nm * 6;
+  const int items = nx * cols;
+  for (int e = tid; e < items; e += kThreads) {
+    const int x = e / cols, t = e - x * cols;
+    const long long q = (long long)m0 * 6 + t;
+    lds[e] = (q < colmax) ? S[(long long)x * colmax + q] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  fft_lines<false, true, E>(lds, g.px, cols, cols, 1);
+
+  double acc = 0.0;
+  const int nmodes = nx * Tm;
+  for (int e = tid; e < nmodes; e += kThreads) {
+    const int x = e / Tm, m = e - x * Tm;
+    const long long gm = (long long)m0 + m;
+    if (gm >= nm) continue;
+    const int ky = (int)(gm / P), kz = (int)(gm - (long long)ky * P);
+    float2* t = lds + x * cols + 6 * m;
+    float2 s[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) s[c] = t[c];
+    if (x == 0 && gm == 0) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) dc[r * 6 + c] = s[c].x;
+    }
+    const float sxk = g.sx[x], cxk = g.cx[x];
+    const float syk = g.sy[ky], cyk = g.cy[ky];
+    const float szk = g.sz[kz], czk = g.cz[kz];
+    const float k0 = g.gx * sxk * cyk * czk;
+    const float k1 = g.gy * cxk * syk * czk;
+    const float k2 = g.gz * cxk * cyk * szk;
+    // d = sigma_hat . xi~
+    float2 d0, d1, d2;
+    d0.x = k0 * s[0].x + k1 * s[3].x + k2 * s[4].x;
+    d0.y = k0 * s[0].y + k1 * s[3].y + k2 * s[4].y;
+    d1.x = k0 * s[3].x + k1 * s[1].x + k2 * s[5].x;
+    d1.y = k0 * s[3].y + k1 * s[1].y + k2 * s[5].y;
+    d2.x = k0 * s[4].x + k1 * s[5].x + k2 * s[2].x;
+    d2.y = k0 * s[4].y + k1 * s[5].y + k2 * s[2].y;
+    const double wk = (kz == 0 || 2 * kz == g.nz) ? 1.0 : 2.0;
+    acc += wk * ((double)d0.x * d0.x + (double)d0.y * d0.y + (double)d1.x * d1.x + (double)d1.y * d1.y +
+                 (double)d2.x * d2.x + (double)d2.y * d2.y);
+    if (do_gamma) {
+      const float q2 = k0 * k0 + k1 * k1 + k2 * k2;
+      float2 o[6];
+      if (q2 > 0.f) {
+        const float inv = 1.0f / sqrtf(q2);
+        const float n0 = k0 * inv, n1 = k1 * inv, n2 = k2 * inv;
+        const float2 v0 = cscale(d0, inv), v1 = cscale(d1, inv), v2 = cscale(d2, inv);
+        const float2 rho = make_float2(n0 * v0.x + n1 * v1.x + n2 * v2.x, n0 * v0.y + n1 * v1.y + n2 * v2.y);
+        const float cA = d.cA, cB = d.cB;
+        const float h = 0.5f * cA;
+        o[0] = make_float2(cA * n0 * v0.x + cB * rho.x * n0 * n0, cA * n0 * v0.y + cB * rho.y * n0 * n0);
+        o[1] = make_float2(cA * n1 * v1.x + cB * rho.x * n1 * n1, cA * n1 * v1.y + cB * rho.y * n1 * n1);
+        o[2] = make_float2(cA * n2 * v2.x + cB * rho.x * n2 * n2, cA * n2 * v2.y + cB * rho.y * n2 * n2);
+        o[3] = make_float2(h * (n0 * v1.x + n1 * v0.x) + cB * rho.x * n0 * n1,
+                           h * (n0 * v1.y + n1 * v0.y) + cB * rho.y * n0 * n1);
+        o[4] = make_float2(h * (n0 * v2.x + n2 * v0.x) + cB * rho.x * n0 * n2,
+                           h * (n0 * v2.y + n2 * v0.y) + cB * rho.y * n0 * n2);
+        o[5] = make_float2(h * (n1 * v2.x + n2 * v1.x) + cB * rho.x * n1 * n2,
+                           h * (n1 * v2.y + n2 * v1.y) + cB * rho.y * n1 * n2);
+      } else {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) o[c] = make_float2(0.f, 0.f);
+      }
+#pragma unroll
+      for (int c = 0; c < 6; ++c) t[c] = o[c];
+    }
+  }
+  const double bs = block_sum(acc, red);
+  if (tid == 0) partials[(long long)r * gridDim.x + blockIdx.x] = bs;
+  __syncthreads();
+  if (!do_gamma) return;
+  fft_lines<true, true, E>(lds, g.px, cols, cols, 1);
+  for (int e = tid; e < items; e += kThreads) {
+    const int x = e / cols, t = e - x * cols;
+    const long long q = (long long)m0 * 6 + t;
+    if (q < colmax) S[(long long)x * colmax + q] = lds[e];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Stop test per load case: R = sqrt(sum partials) / ||sigma_hat(0)||_V
+// (lippmann_schwinger.py:118-143; stop rule of lippmann_schwinger.cu:205:
+// R < max(rtol R0, atol), R0 = residual of the first check unless preset).
+__global__ void __launch_bounds__(kThreads) k_check(RhsState* __restrict__ st, const double* __restrict__ partials,
+                                                    const float* __restrict__ dc, int nparts, double rtol,
+                                                    double atol, int record_only) {
+  __shared__ double red[kThreads / 64];
+  const int r = blockIdx.x;
+  if (!st[r].active) return;
+  const double* p = partials + (long long)r * nparts;
+  double a = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += kThreads) a += p[i];
+  const double s = block_sum(a, red);
+  if (threadIdx.x == 0) {
+    const float* z = dc + r * 6;
+    const double den = sqrt((double)z[0] * z[0] + (double)z[1] * z[1] + (double)z[2] * z[2] +
+                            2.0 * ((double)z[3] * z[3] + (double)z[4] * z[4] + (double)z[5] * z[5]));
+    const double R = sqrt(s) / den;
+    RhsState q = st[r];
+    q.R = R;
+    q.den = den;
+    if (!(R == R) || R > 1e300) {
+      q.status = 1;
+      q.active = 0;
+    } else if (!record_only) {
+      if (q.checks == 0 && !(q.R0 > 0.0)) q.R0 = R;
+      const double thr = fmax(rtol * q.R0, atol);
+      if (R < thr) {
+        q.converged = 1;
+        q.active = 0;
+      }
+    }
+    q.checks += 1;
+    st[r] = q;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// sigma = C eps (+tau), optional store, FP64 per-block component sums.
+__global__ void __launch_bounds__(kThreads) k_stress(GridDev g, const RhsDesc* __restrict__ rhs,
+                                                     const int* __restrict__ sel, const float* __restrict__ lam,
+                                                     const float* __restrict__ mu, const float* __restrict__ eps,
+                                                     const float* __restrict__ tau, float* __restrict__ sigma,
+                                                     double* __restrict__ part6, int chunk) {
+  __shared__ double red[kThreads / 64];
+  const int r = blockIdx.y;
+  if (sel && !sel[r]) return;
+  const RhsDesc d = rhs[r];
+  const long long N = g.N;
+  const float* E_ = eps + (long long)r * 6 * N;
+  const float* T_ = tau ? tau + (long long)r * 6 * N : nullptr;
+  float* S_ = sigma ? sigma + (long long)r * 6 * N : nullptr;
+  const float* LA = lam + (long long)d.mat * N;
+  const float* MU = mu + (long long)d.mat * N;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  const long long v0 = (long long)blockIdx.x * chunk;
+  const long long v1 = min(N, v0 + chunk);
+  for (long long v = v0 + threadIdx.x; v < v1; v += kThreads) {
+    float e[6], s[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) e[c] = E_[c * N + v];
+    hooke(e, LA[v], MU[v], s);
+    if (T_) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) s[c] += T_[c * N + v];
+    }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      if (S_) S_[c * N + v] = s[c];
+      acc[c] += s[c];
+    }
+  }
+  for (int c = 0; c < 6; ++c) {
+    const double bs = block_sum(acc[c], red);
+    if (threadIdx.x == 0 && part6) part6[((long long)r * gridDim.x + blockIdx.x) * 6 + c] = bs;
+  }
+}
+
+// Sum the per-block partials of k_stress (fixed order) into out[r][6].
+__global__ void __launch_bounds__(kThreads) k_sum6(const double* __restrict__ part6, int nparts,
+                                                   double* __restrict__ out, double scale) {
+  __shared__ double red[kThreads / 64];
+  const int r = blockIdx.x;
+  for (int c = 0; c < 6; ++c) {
+    double a = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += kThreads) a += part6[((long long)r * nparts + i) * 6 + c];
+    const double s = block_sum(a, red);
+    if (threadIdx.x == 0) out[r * 6 + c] = s * scale;
+  }
+}
+
+// min/max of the Lame fields of one material (reference medium,
+// lippmann_schwinger.cu:172-173): out[mat] = {min lam, max lam, min mu, max mu}
+__global__ void __launch_bounds__(kThreads) k_minmax(const float* __restrict__ lam, const float* __restrict__ mu,
+                                                     long long N, float* __restrict__ part, int chunk) {
+  __shared__ float red[4][kThreads];
+  const int mat = blockIdx.y;
+  const float* LA = lam + (long long)mat * N;
+  const float* MU = mu + (long long)mat * N;
+  float a = 3.4e38f, b = -3.4e38f, c = 3.4e38f, dd = -3.4e38f;
+  const long long v0 = (long long)blockIdx.x * chunk;
+  const long long v1 = min(N, v0 + chunk);
+  for (long long v = v0 + threadIdx.x; v < v1; v += kThreads) {
+    const float l = LA[v], m = MU[v];
+    a = fminf(a, l);
+    b = fmaxf(b, l);
+    c = fminf(c, m);
+    dd = fmaxf(dd, m);
+  }
+  red[0][threadIdx.x] = a;
+  red[1][threadIdx.x] = b;
+  red[2][threadIdx.x] = c;
+  red[3][threadIdx.x] = dd;
+  __syncthreads();
+  for (int s = kThreads / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      red[0][threadIdx.x] = fminf(red[0][threadIdx.x], red[0][threadIdx.x + s]);
+      red[1][threadIdx.x] = fmaxf(red[1][threadIdx.x], red[1][threadIdx.x + s]);
+      red[2][threadIdx.x] = fminf(red[2][threadIdx.x], red[2][threadIdx.x + s]);
+      red[3][threadIdx.x] = fmaxf(red[3][threadIdx.x], red[3][threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    float* o = part + ((long long)mat * gridDim.x + blockIdx.x) * 4;
+    o[0] = red[0][0];
+    o[1] = red[1][0];
+    o[2] = red[2][0];
+    o[3] = red[3][0];
+  }
+}
+
+// Anderson step per load case (lippmann_schwinger.py:217-228): reduce the new
+// Gram row <r, r^(s)>_w (FP64, fixed order), shift the Gram matrix by one age,
+// solve A v = u on the filled block by Gaussian elimination with partial
+// pivoting in FP64, alpha = v / (u . v).  A singular or non-finite system falls
+// back to the plain step alpha = e_0.  One block per load case.
+__global__ void __launch_bounds__(kThreads) k_aa_solve(const RhsState* __restrict__ st,
+                                                       const double* __restrict__ dots, int nblk, int m1,
+                                                       double* __restrict__ gram, double* __restrict__ alpha,
+                                                       const int* __restrict__ head,
+                                                       const int* __restrict__ filled) {
+  __shared__ double red[kThreads / 64];
+  __shared__ double row[kMaxDepth1];
+  const int r = blockIdx.x;
+  if (!st[r].active) return;
+  const int f = filled[r];
+  for (int s = 0; s < m1; ++s) {
+    double a = 0.0;
+    for (int i = threadIdx.x; i < nblk; i += kThreads) a += dots[((long long)r * nblk + i) * m1 + s];
+    const double bs = block_sum(a, red);
+    if (threadIdx.x == 0) row[s] = (s < f) ? bs : 0.0;
+  }
+  if (threadIdx.x != 0) return;
+  double* A = gram + (long long)r * m1 * m1;
+  for (int i = m1 - 1; i >= 1; --i)
+    for (int j = m1 - 1; j >= 1; --j) A[i * m1 + j] = A[(i - 1) * m1 + (j - 1)];
+  for (int s = 0; s < m1; ++s) {
+    A[s] = row[s];
+    A[s * m1] = row[s];
+  }
+  for (int s = f; s < m1; ++s) A[s * m1 + s] = 1.0;  // unfilled ages keep the identity
+  double M[kMaxDepth1][kMaxDepth1 + 1];
+  for (int i = 0; i < f; ++i) {
+    for (int j = 0; j < f; ++j) M[i][j] = A[i * m1 + j];
+    M[i][f] = 1.0;
+  }
+  bool ok = true;
+  for (int c = 0; c < f && ok; ++c) {
+    int piv = c;
+    for (int i = c + 1; i < f; ++i)
+      if (fabs(M[i][c]) > fabs(M[piv][c])) piv = i;
+    if (!(fabs(M[piv][c]) > 1e-300)) { ok = false; break; }
+    if (piv != c)
+      for (int j = c; j <= f; ++j) { const double t = M[c][j]; M[c][j] = M[piv][j]; M[piv][j] = t; }
+    for (int i = c + 1; i < f; ++i) {
+      const double q = M[i][c] / M[c][c];
+      for (int j = c; j <= f; ++j) M[i][j] -= q * M[c][j];
+    }
+  }
+  double v[kMaxDepth1];
+  double sv = 0.0;
+  if (ok) {
+    for (int i = f - 1; i >= 0; --i) {
+      double t = M[i][f];
+      for (int j = i + 1; j < f; ++j) t -= M[i][j] * v[j];
+      v[i] = t / M[i][i];
+      sv += v[i];
+    }
+    ok = (sv == sv) && fabs(sv) > 1e-300 && fabs(sv) < 1e300;
+  }
+  const int h = head[r];
+  double* al = alpha + (long long)r * m1;
+  for (int s = 0; s < m1; ++s) al[s] = 0.0;
+  for (int s = 0; s < f; ++s) {
+    int sl = h - s;
+    if (sl < 0) sl += m1;
+    al[sl] = ok ? v[s] / sv : (s == 0 ? 1.0 : 0.0);
+  }
+}
+
+// Advance the Anderson ring for every active load case before a step.
+__global__ void k_aa_advance(const RhsState* __restrict__ st, int* __restrict__ head, int* __restrict__ filled,
+                             int nrhs, int m1) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrhs || !st[r].active) return;
+  head[r] = (head[r] + 1) % m1;
+  filled[r] = min(filled[r] + 1, m1);
+}
+
+}  // namespace lsk

@@ -1,0 +1,5 @@
+"""Solver entry points."""
+
+from .lippmann_schwinger import lippmann_schwinger_cuda, effective_tensor  # noqa: F401
+
+__all__ = ["lippmann_schwinger_cuda", "effective_tensor"]

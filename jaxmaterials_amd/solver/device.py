@@ -1,0 +1,124 @@
+"""Device-memory solver for PyTorch tensors (ROCm), over the C ABI of
+liblippmannschwinger.so (``ls_solver_*`` in include/lippmann_schwinger.h).
+
+Torch is used for device memory only; all arithmetic of the solve runs in the
+library's HIP kernels.  There is no CPU fallback: a missing library or a tensor
+that is not on the GPU raises.
+"""
+
+import ctypes
+
+import numpy as np
+import torch
+
+from .. import _lib
+
+__all__ = ["LSSolver", "unit_load_cases"]
+
+
+def unit_load_cases():
+    """The six unit Voigt strains e_0..e_5 (rows), float32 [6, 6]."""
+    return np.eye(6, dtype=np.float32)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _check_dev(name, t, shape=None):
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise RuntimeError(f"{name} must be a GPU tensor (no CPU path exists)")
+    if t.dtype != torch.float32 or not t.is_contiguous():
+        raise TypeError(f"{name} must be contiguous float32")
+    if shape is not None and tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name} has shape {tuple(t.shape)}, expected {tuple(shape)}")
+
+
+class LSSolver:
+    """Batched solver: ``nrhs`` load cases over ``nmat`` Lame-field pairs on one grid.
+
+    :arg grid_spec: GridSpec (nx, ny, nz, Lx, Ly, Lz)
+    :arg nrhs: number of load cases solved together
+    :arg nmat: number of (lambda, mu) pairs
+    :arg depth: Anderson depth (0 = plain Lippmann-Schwinger iteration)
+    """
+
+    def __init__(self, grid_spec, nrhs=6, nmat=1, depth=0):
+        self.lib = _lib.load()
+        self.gs = grid_spec
+        self.shape = (grid_spec.nx, grid_spec.ny, grid_spec.nz)
+        self.nrhs, self.nmat, self.depth = int(nrhs), int(nmat), int(depth)
+        self._vox = np.array(self.shape, dtype=np.int32)
+        self._ext = np.array([grid_spec.Lx, grid_spec.Ly, grid_spec.Lz], dtype=np.float32)
+        h = self.lib.ls_solver_create(
+            self._vox.ctypes.data_as(ctypes.c_void_p), self._ext.ctypes.data_as(ctypes.c_void_p),
+            self.nrhs, self.nmat, self.depth, None,
+        )
+        if not h:
+            raise RuntimeError("ls_solver_create failed (see stderr)")
+        self._h = ctypes.c_void_p(h)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.ls_solver_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def solve(self, lmbda, mu, epsilon_bar, rhs_mat=None, tau=None, epsilon=None, warm_start=False,
+              want_sigma=False, rtol=1e-6, atol=0.0, maxiter=1000, rel_ref=0.0, verbose=0, check=True):
+        """Solve all load cases.
+
+        :arg lmbda, mu: GPU float32 [nmat, nx, ny, nz] (or [nx, ny, nz] when nmat == 1)
+        :arg epsilon_bar: array-like [nrhs, 6] of macroscopic strains
+        :arg rhs_mat: material index of each load case (default all 0)
+        :arg tau: optional GPU float32 [nrhs, 6, nx, ny, nz] polarisation added to the stress
+        :arg epsilon: optional GPU float32 [nrhs, 6, nx, ny, nz]; start field if warm_start
+        :arg rel_ref: normaliser of rtol; 0 means the residual of the uniform start
+        Returns a dict with epsilon, sigma (or None), iters, residual, sigma_mean (float64 [nrhs, 6]).
+        """
+        R = self.nrhs
+        field = (R, 6) + self.shape
+        lm = lmbda.reshape((self.nmat,) + self.shape) if lmbda.dim() == 3 else lmbda
+        m = mu.reshape((self.nmat,) + self.shape) if mu.dim() == 3 else mu
+        _check_dev("lmbda", lm, (self.nmat,) + self.shape)
+        _check_dev("mu", m, (self.nmat,) + self.shape)
+        eb = np.ascontiguousarray(np.asarray(epsilon_bar, dtype=np.float32).reshape(R, 6))
+        mats = np.zeros(R, dtype=np.int32) if rhs_mat is None else np.ascontiguousarray(rhs_mat, dtype=np.int32)
+        if tau is not None:
+            _check_dev("tau", tau, field)
+        if epsilon is None:
+            if warm_start:
+                raise ValueError("warm_start needs epsilon")
+            epsilon = torch.empty(field, dtype=torch.float32, device=lm.device)
+        _check_dev("epsilon", epsilon, field)
+        sigma = torch.empty(field, dtype=torch.float32, device=lm.device) if want_sigma else None
+        iters = np.zeros(R, dtype=np.int32)
+        res = np.zeros(R, dtype=np.float64)
+        smean = np.zeros((R, 6), dtype=np.float64)
+        torch.cuda.synchronize(lm.device)
+        rc = self.lib.ls_solver_solve(
+            self._h, _ptr(lm), _ptr(m), mats.ctypes.data_as(ctypes.c_void_p), eb.ctypes.data_as(ctypes.c_void_p),
+            _ptr(tau), _ptr(epsilon), _ptr(sigma), int(bool(warm_start)), float(rtol), float(atol), int(maxiter),
+            float(rel_ref), int(verbose), iters.ctypes.data_as(ctypes.c_void_p),
+            res.ctypes.data_as(ctypes.c_void_p), smean.ctypes.data_as(ctypes.c_void_p),
+        )
+        if rc < 0:
+            raise RuntimeError("ls_solver_solve failed (see stderr)")
+        if check and rc == 1:
+            raise RuntimeError(f"Solver failed to converge after {maxiter} iterations")
+        return {"epsilon": epsilon, "sigma": sigma, "iters": iters, "residual": res, "sigma_mean": smean,
+                "converged": rc == 0}
+
+    def last_timing(self):
+        """Kernel milliseconds of the last solve by pass kind (z, y, x, other) when
+        the environment variable LS_TIMING was set, and the pass counts."""
+        ms = np.zeros(4, dtype=np.float64)
+        cnt = np.zeros(4, dtype=np.int32)
+        self.lib.ls_solver_last_timing(self._h, ms.ctypes.data_as(ctypes.c_void_p),
+                                       cnt.ctypes.data_as(ctypes.c_void_p))
+        return ms, cnt

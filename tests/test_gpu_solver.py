@@ -1,0 +1,196 @@
+"""GPU parity tests of the HIP solver against the FP64 oracle.
+
+Grids and inputs follow the reference's own tests:
+* tests/test_lippmann_schwinger.py:18-47 (even [64,48,32] / odd [57,43,37],
+  L = (1.2, 0.8, 0.9), lambda, mu ~ U(0.8, 1.2), seed 784173);
+* cuda/src/tests/test_lippmann_schwinger.hh:19-36, 104-180 (even 48x64x32 /
+  odd 47x61x37, L = (1.1, 0.9, 0.7)).
+Tolerances (the bar): strain relative L2 <= 1e-5 against the FP64 oracle at the
+same number of updates, iteration count within +-1, C_eff within 1e-5 relative.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from jaxmaterials_amd.common import GridSpec
+from jaxmaterials_amd.solver.lippmann_schwinger import lippmann_schwinger_cuda, effective_tensor
+from jaxmaterials_amd.solver.device import LSSolver, unit_load_cases
+from oracle import ls_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+PY_GRIDS = {"even": (64, 48, 32), "odd": (57, 43, 37)}
+CU_GRIDS = {"even": (48, 64, 32), "odd": (47, 61, 37)}
+EPS_BAR = np.array([2.1, 0.9, 0.8, 0.4, 0.9, 0.5])
+
+
+def py_grid(name):
+    return GridSpec(*PY_GRIDS[name], 1.2, 0.8, 0.9)
+
+
+def cu_grid(name):
+    return GridSpec(*CU_GRIDS[name], 1.1, 0.9, 0.7)
+
+
+def random_material(gs, seed=784173, lo=0.8, hi=1.2):
+    """initialise_material of tests/test_lippmann_schwinger.py:37-47 (returns mu, lambda)."""
+    rng = np.random.default_rng(seed=seed)
+    shape = (gs.nx, gs.ny, gs.nz)
+    mu = np.ones(shape) + rng.uniform(size=shape, low=lo - 1.0, high=hi - 1.0)
+    lmbda = np.ones(shape) + rng.uniform(size=shape, low=lo - 1.0, high=hi - 1.0)
+    return mu.astype(np.float32), lmbda.astype(np.float32)
+
+
+def rel_l2(a, b):
+    return float(np.sqrt(np.sum((np.asarray(a, np.float64) - b) ** 2) / np.sum(np.asarray(b, np.float64) ** 2)))
+
+
+def oracle_fixed_updates(lmbda, mu, eps_bar, gs, k, depth=0, tau=None):
+    """FP64 oracle strain after exactly k updates."""
+    eps, sigma, it, hist = O.lippmann_schwinger_rfft(
+        lmbda.astype(np.float64), mu.astype(np.float64), eps_bar, gs, rtol=0.0, atol=0.0, depth=depth,
+        maxiter=k, tau=tau)
+    assert it == k
+    return eps, sigma, hist
+
+
+@pytest.mark.parametrize("grid", ["even", "odd"])
+def test_homogeneous_material_zero_iterations(gpu, grid):
+    """Known answer of TestHomogeneousMaterial (cuda/src/tests/test_lippmann_schwinger.hh:104-128)."""
+    gs = cu_grid(grid)
+    shape = (gs.nx, gs.ny, gs.nz)
+    lam = np.full(shape, 1.2, np.float32)
+    mu = np.full(shape, 1.2, np.float32)
+    eb = np.array([1.0, 0.4, 0.3, 0.1, -0.4, 0.7], np.float32)
+    eps, sigma, it = lippmann_schwinger_cuda(lam, mu, eb, gs, rtol=1e-20, atol=1e-4)
+    print(f"homogeneous {grid}: iter={it}", flush=True)
+    assert it == 0
+    assert np.allclose(eps, eb[:, None, None, None], rtol=0, atol=1e-7)
+    s = O.compute_sigma(lam.astype(np.float64), mu.astype(np.float64), np.broadcast_to(
+        eb[:, None, None, None].astype(np.float64), (6,) + shape).copy())
+    assert np.allclose(sigma, s, rtol=1e-6, atol=1e-6)
+
+
+@pytest.mark.parametrize("grid", ["even", "odd"])
+def test_convergence_cuda_test(gpu, grid):
+    """TestConvergence (cuda/src/tests/test_lippmann_schwinger.hh:132-180): iter < 10 and
+    ||D- sigma|| / ||<sigma>||_V < 1e-2 for lambda, mu ~ U(0.8, 0.9)."""
+    gs = cu_grid(grid)
+    rng = np.random.default_rng(7812481)
+    shape = (gs.nx, gs.ny, gs.nz)
+    mu = rng.uniform(0.8, 0.9, size=shape).astype(np.float32)
+    lam = rng.uniform(0.8, 0.9, size=shape).astype(np.float32)
+    eb = np.array([1.0, 0.4, 0.3, 0.1, -0.4, 0.7], np.float32)
+    eps, sigma, it = lippmann_schwinger_cuda(lam, mu, eb, gs, rtol=1e-20, atol=1e-4)
+    div = O.backward_divergence(sigma.astype(np.float64), gs)
+    avg = sigma.astype(np.float64).mean(axis=(1, 2, 3))
+    ratio = np.sqrt(np.sum(div**2)) / np.sqrt(np.sum(O.VOIGT_WEIGHTS * avg**2))
+    print(f"convergence {grid}: iter={it} ratio={ratio:.3e}", flush=True)
+    assert it < 10
+    assert ratio < 1e-2
+
+
+@pytest.mark.parametrize("grid", ["even", "odd"])
+def test_legacy_abi_matches_oracle(gpu, grid):
+    """test_jax_matches_cuda (tests/test_lippmann_schwinger.py:102-141) with the FP64
+    oracle in place of JAX and the bar tightened to 1e-5 relative L2 of the strain."""
+    gs = py_grid(grid)
+    lmbda, mu = random_material(gs)  # reference quirk Q2: the names are swapped there too
+    atol = 1e-5
+    eps, sigma, it = lippmann_schwinger_cuda(lmbda, mu, EPS_BAR.astype(np.float32), gs, rtol=1e-20,
+                                             atol=atol, maxiter=32)
+    # the loader passes mu in the lambda slot (lippmann_schwinger.py:310-312)
+    e_ref, s_ref, it_ref = O.lippmann_schwinger_capi(mu, lmbda, EPS_BAR, gs, rtol=1e-20, atol=atol, maxiter=32)
+    e_k, s_k, _ = oracle_fixed_updates(mu, lmbda, EPS_BAR, gs, it)
+    err_e = rel_l2(eps, e_k)
+    err_s = rel_l2(sigma, s_k)
+    print(f"legacy {grid}: iter gpu={it} oracle={it_ref} strain relL2={err_e:.2e} stress relL2={err_s:.2e}",
+          flush=True)
+    assert abs(it - it_ref) <= 1
+    assert err_e <= 1e-5
+    assert err_s <= 1e-5
+
+
+@pytest.mark.parametrize("grid", ["even", "odd"])
+def test_gamma0_divergence_free(gpu, grid):
+    """TestDivSigma invariant (cuda/src/tests/test_fourier.hh:267-316) through the solver:
+    in a homogeneous medium with a random polarisation tau, one update gives
+    sigma = C0 eps + tau divergence free."""
+    gs = cu_grid(grid)
+    shape = (gs.nx, gs.ny, gs.nz)
+    rng = np.random.default_rng(12)
+    tau = rng.normal(size=(1, 6) + shape).astype(np.float32)
+    lam = torch.full(shape, 0.8, device=gpu)
+    mu = torch.full(shape, 1.3, device=gpu)
+    s = LSSolver(gs, nrhs=1)
+    out = s.solve(lam, mu, np.zeros((1, 6)), tau=torch.from_numpy(tau).to(gpu), want_sigma=True,
+                  rtol=0.0, atol=0.0, maxiter=1, check=False)
+    sig = out["sigma"].cpu().numpy()[0].astype(np.float64)
+    div = O.backward_divergence(sig, gs)
+    rel = np.sqrt(np.sum(div**2)) / np.sqrt(np.sum(O.VOIGT_WEIGHTS[:, None, None, None] * sig**2))
+    e_k, s_k, _ = oracle_fixed_updates(np.full(shape, 0.8), np.full(shape, 1.3), np.zeros(6), gs, 1,
+                                       tau=tau[0].astype(np.float64))
+    err = rel_l2(out["epsilon"].cpu().numpy()[0], e_k)
+    print(f"divsigma {grid}: rel div={rel:.2e} strain relL2 vs oracle={err:.2e}", flush=True)
+    assert rel < 5e-5
+    assert err < 1e-5
+
+
+@pytest.mark.parametrize("depth", [0, 2, 4])
+@pytest.mark.parametrize("grid", ["even", "odd"])
+def test_batched_anderson_matches_oracle(gpu, grid, depth):
+    """test_convergence (tests/test_lippmann_schwinger.py:64-99) settings, all through the
+    batched device solver: iteration bounds, and strain parity with the FP64 oracle."""
+    gs = py_grid(grid)
+    lmbda, mu = random_material(gs)
+    lam_t = torch.from_numpy(lmbda).to(gpu)
+    mu_t = torch.from_numpy(mu).to(gpu)
+    s = LSSolver(gs, nrhs=2, depth=depth)
+    eb = np.stack([EPS_BAR, -0.5 * EPS_BAR[::-1]]).astype(np.float32)
+    out = s.solve(lam_t, mu_t, eb, rtol=1e-20, atol=1e-5, maxiter=32, want_sigma=True)
+    its = out["iters"]
+    for r in range(2):
+        e_ref, s_ref, it_ref, _ = O.lippmann_schwinger_rfft(lmbda.astype(np.float64), mu.astype(np.float64),
+                                                            eb[r].astype(np.float64), gs, rtol=1e-20, atol=1e-5,
+                                                            depth=depth, maxiter=32)
+        e_k, s_k, _ = oracle_fixed_updates(lmbda, mu, eb[r].astype(np.float64), gs, int(its[r]), depth=depth)
+        err = rel_l2(out["epsilon"][r].cpu().numpy(), e_k)
+        print(f"AA depth={depth} {grid} rhs{r}: iter gpu={its[r]} oracle={it_ref} strain relL2={err:.2e}",
+              flush=True)
+        assert its[r] < 8  # float32 bound of test_convergence
+        assert abs(int(its[r]) - it_ref) <= 1
+        assert err <= 1e-5
+
+
+def test_effective_tensor_config1(gpu):
+    """32^3 cross-ply (4 layers x 8 voxels, r = 2.5, vf 0.4 RSA): C_eff from one batched
+    solve of the six load cases against the FP64 oracle."""
+    from jaxmaterials_amd.data import crossply_lame
+
+    shape = (32, 32, 32)
+    gs = GridSpec(*shape, 32.0, 32.0, 32.0)
+    lam, mu, vf, nf = crossply_lame(shape, (32.0, 32.0, 32.0), 4, 2.5, 0.4, seed=1)
+    lam_np, mu_np = lam.cpu().numpy(), mu.cpu().numpy()
+    C, its = effective_tensor(lam_np, mu_np, gs, rtol=1e-6, maxiter=2000)
+    C_ref, its_ref = O.effective_tensor(lam_np.astype(np.float64), mu_np.astype(np.float64), gs, rtol=1e-6)
+    rel = np.abs(C - C_ref).max() / np.abs(C_ref).max()
+    print(f"config1 C_eff: vf={vf:.3f} fibres={nf} iters gpu={its.tolist()} oracle={its_ref} "
+          f"max rel err={rel:.2e}", flush=True)
+    assert rel <= 1e-5
+    assert all(abs(int(a) - b) <= 1 for a, b in zip(its, its_ref))
+
+
+def test_rasteriser_matches_oracle(gpu):
+    from jaxmaterials_amd.data import crossply_lame
+    from oracle import microstructure_oracle as MO
+
+    for shape, nl, r, vf, seed in [((32, 32, 32), 4, 2.5, 0.4, 1), ((48, 40, 36), 3, 4.0, 0.5, 5),
+                                   ((64, 64, 64), 8, 5.0, 0.6, 9)]:
+        ext = tuple(float(n) for n in shape)
+        lam, mu, avf, nf = crossply_lame(shape, ext, nl, r, vf, seed=seed)
+        lam_r, mu_r, vf_r, fib = MO.crossply(shape, ext, nl, r, vf, seed)
+        assert nf == len(fib)
+        assert np.array_equal(lam.cpu().numpy(), lam_r)
+        assert np.array_equal(mu.cpu().numpy(), mu_r)
+        assert abs(avf - vf_r) < 1e-12
