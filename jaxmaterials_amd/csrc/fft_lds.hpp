@@ -9,6 +9,9 @@
 // direct O(n^2) DFT per line (the reference accepts any grid size; its tests
 // use 37, 43, 47, 61 and 57 = 3 * 19).
 //
+// The engine is templated on the complex type: float2 for the iteration,
+// double2 for the rare FP64 base evaluations (tw64: FP64 table).
+//
 // Forward is unnormalised with exp(-i ...), inverse is unnormalised with
 // exp(+i ...), matching the FFT conventions of the reference
 // (src/jaxmaterials/solver/lippmann_schwinger.py:192, 216; cuFFT R2C/C2R at
@@ -25,55 +28,73 @@ struct AxisPlan {
   int n;
   int nstages;  // 0 => direct DFT
   int radix[kMaxStages];
-  const float2* tw;  // n entries, exp(-2 pi i t / n)
+  const float2* tw;    // n entries, exp(-2 pi i t / n)
+  const double2* tw64;  // the same in FP64
 };
 
-__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
-  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
+template <typename V>
+struct CT;
+template <>
+struct CT<float2> {
+  using S = float;
+  static __device__ __forceinline__ const float2* tw(const AxisPlan& p) { return p.tw; }
+  static __device__ __forceinline__ float2 mk(float x, float y) { return make_float2(x, y); }
+};
+template <>
+struct CT<double2> {
+  using S = double;
+  static __device__ __forceinline__ const double2* tw(const AxisPlan& p) { return p.tw64; }
+  static __device__ __forceinline__ double2 mk(double x, double y) { return make_double2(x, y); }
+};
+
+template <typename V>
+__device__ __forceinline__ V cadd(V a, V b) { return CT<V>::mk(a.x + b.x, a.y + b.y); }
+template <typename V>
+__device__ __forceinline__ V csub(V a, V b) { return CT<V>::mk(a.x - b.x, a.y - b.y); }
+template <typename V>
+__device__ __forceinline__ V cmul(V a, V b) { return CT<V>::mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
 // a * conj(b)
-__device__ __forceinline__ float2 cmulc(float2 a, float2 b) {
-  return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
-}
-__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+template <typename V>
+__device__ __forceinline__ V cmulc(V a, V b) { return CT<V>::mk(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y); }
+template <typename V>
+__device__ __forceinline__ V cscale(V a, typename CT<V>::S s) { return CT<V>::mk(a.x * s, a.y * s); }
 // multiply by -i (forward) or +i (inverse)
-template <bool INV>
-__device__ __forceinline__ float2 mul_mi(float2 a) {
-  return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+template <bool INV, typename V>
+__device__ __forceinline__ V mul_mi(V a) {
+  return INV ? CT<V>::mk(-a.y, a.x) : CT<V>::mk(a.y, -a.x);
 }
 
-template <bool INV>
-__device__ __forceinline__ void dft2(float2* v) {
-  float2 a = v[0], b = v[1];
+template <bool INV, typename V>
+__device__ __forceinline__ void dft2(V* v) {
+  V a = v[0], b = v[1];
   v[0] = cadd(a, b);
   v[1] = csub(a, b);
 }
 
-template <bool INV>
-__device__ __forceinline__ void dft4(float2* v) {
-  float2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
-  float2 t2 = cadd(v[1], v[3]), t3 = mul_mi<INV>(csub(v[1], v[3]));
+template <bool INV, typename V>
+__device__ __forceinline__ void dft4(V* v) {
+  V t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+  V t2 = cadd(v[1], v[3]), t3 = mul_mi<INV>(csub(v[1], v[3]));
   v[0] = cadd(t0, t2);
   v[2] = csub(t0, t2);
   v[1] = cadd(t1, t3);
   v[3] = csub(t1, t3);
 }
 
-template <bool INV>
-__device__ __forceinline__ void dft8(float2* v) {
-  constexpr float s = 0.70710678118654752440f;
-  float2 e[4] = {v[0], v[2], v[4], v[6]};
-  float2 o[4] = {v[1], v[3], v[5], v[7]};
+template <bool INV, typename V>
+__device__ __forceinline__ void dft8(V* v) {
+  using S = typename CT<V>::S;
+  constexpr S s = S(0.70710678118654752440);
+  V e[4] = {v[0], v[2], v[4], v[6]};
+  V o[4] = {v[1], v[3], v[5], v[7]};
   dft4<INV>(e);
   dft4<INV>(o);
   // W8^k for k = 1, 2, 3 (conjugated for the inverse)
-  float2 o1 = INV ? make_float2(s * (o[1].x - o[1].y), s * (o[1].x + o[1].y))
-                  : make_float2(s * (o[1].x + o[1].y), s * (o[1].y - o[1].x));
-  float2 o2 = mul_mi<INV>(o[2]);
-  float2 o3 = INV ? make_float2(-s * (o[3].x + o[3].y), s * (o[3].x - o[3].y))
-                  : make_float2(s * (o[3].y - o[3].x), -s * (o[3].x + o[3].y));
+  V o1 = INV ? CT<V>::mk(s * (o[1].x - o[1].y), s * (o[1].x + o[1].y))
+             : CT<V>::mk(s * (o[1].x + o[1].y), s * (o[1].y - o[1].x));
+  V o2 = mul_mi<INV>(o[2]);
+  V o3 = INV ? CT<V>::mk(-s * (o[3].x + o[3].y), s * (o[3].x - o[3].y))
+             : CT<V>::mk(s * (o[3].y - o[3].x), -s * (o[3].x + o[3].y));
   v[0] = cadd(e[0], o[0]);
   v[4] = csub(e[0], o[0]);
   v[1] = cadd(e[1], o1);
@@ -84,31 +105,33 @@ __device__ __forceinline__ void dft8(float2* v) {
   v[7] = csub(e[3], o3);
 }
 
-template <bool INV>
-__device__ __forceinline__ void dft3(float2* v) {
-  constexpr float s3 = 0.86602540378443864676f;
-  float2 t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
-  float2 y0 = cadd(v[0], t1);
-  float2 m = make_float2(v[0].x - 0.5f * t1.x, v[0].y - 0.5f * t1.y);
-  float2 it2 = mul_mi<INV>(cscale(t2, s3));  // -i s3 t2 (forward)
+template <bool INV, typename V>
+__device__ __forceinline__ void dft3(V* v) {
+  using S = typename CT<V>::S;
+  constexpr S s3 = S(0.86602540378443864676);
+  V t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
+  V y0 = cadd(v[0], t1);
+  V m = CT<V>::mk(v[0].x - S(0.5) * t1.x, v[0].y - S(0.5) * t1.y);
+  V it2 = mul_mi<INV>(cscale(t2, s3));  // -i s3 t2 (forward)
   v[0] = y0;
   v[1] = cadd(m, it2);
   v[2] = csub(m, it2);
 }
 
-template <bool INV>
-__device__ __forceinline__ void dft5(float2* v) {
-  constexpr float c1 = 0.30901699437494742410f;   // cos(2 pi / 5)
-  constexpr float c2 = -0.80901699437494742410f;  // cos(4 pi / 5)
-  constexpr float s1 = 0.95105651629515357212f;   // sin(2 pi / 5)
-  constexpr float s2 = 0.58778525229247312917f;   // sin(4 pi / 5)
-  float2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
-  float2 t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
-  float2 a1 = make_float2(v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y);
-  float2 a2 = make_float2(v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y);
-  float2 b1 = make_float2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y);
-  float2 b2 = make_float2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y);
-  float2 ib1 = mul_mi<INV>(b1), ib2 = mul_mi<INV>(b2);
+template <bool INV, typename V>
+__device__ __forceinline__ void dft5(V* v) {
+  using S = typename CT<V>::S;
+  constexpr S c1 = S(0.30901699437494742410);   // cos(2 pi / 5)
+  constexpr S c2 = S(-0.80901699437494742410);  // cos(4 pi / 5)
+  constexpr S s1 = S(0.95105651629515357212);   // sin(2 pi / 5)
+  constexpr S s2 = S(0.58778525229247312917);   // sin(4 pi / 5)
+  V t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+  V t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+  V a1 = CT<V>::mk(v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y);
+  V a2 = CT<V>::mk(v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y);
+  V b1 = CT<V>::mk(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y);
+  V b2 = CT<V>::mk(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y);
+  V ib1 = mul_mi<INV>(b1), ib2 = mul_mi<INV>(b2);
   v[0] = cadd(v[0], cadd(t1, t2));
   v[1] = cadd(a1, ib1);
   v[4] = csub(a1, ib1);
@@ -116,8 +139,8 @@ __device__ __forceinline__ void dft5(float2* v) {
   v[3] = csub(a2, ib2);
 }
 
-template <int R, bool INV>
-__device__ __forceinline__ void dftR(float2* v) {
+template <int R, bool INV, typename V>
+__device__ __forceinline__ void dftR(V* v) {
   if constexpr (R == 2) dft2<INV>(v);
   else if constexpr (R == 3) dft3<INV>(v);
   else if constexpr (R == 4) dft4<INV>(v);
@@ -141,28 +164,28 @@ __device__ __forceinline__ void split_item(int w, int nlines, int per_line, int&
 
 // One Stockham stage of radix R, in place in LDS.  E = register elements per
 // thread; the host guarantees ceil(nlines * n / R / kThreads) * R <= E.
-template <int R, bool INV, bool LINE_FAST, int E>
-__device__ __forceinline__ void stockham_stage(float2* buf, int n, int Ns, int nlines, int es, int ls,
-                                               const float2* __restrict__ tw) {
+template <int R, bool INV, bool LINE_FAST, int E, typename V>
+__device__ __forceinline__ void stockham_stage(V* buf, int n, int Ns, int nlines, int es, int ls,
+                                               const V* __restrict__ tw) {
   constexpr int NB = E / R;
   const int tid = threadIdx.x;
   const int nR = n / R;
   const int total = nlines * nR;
   const int tws = n / (Ns * R);
-  float2 v[NB * R];
+  V v[NB * R];
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
     const int w = tid + b * kThreads;
     if (w < total) {
       int l, j;
       split_item<LINE_FAST>(w, nlines, nR, l, j);
-      const float2* src = buf + l * ls;
+      const V* src = buf + l * ls;
       const int k = j % Ns;
 #pragma unroll
       for (int q = 0; q < R; ++q) {
-        float2 x = src[(j + q * nR) * es];
+        V x = src[(j + q * nR) * es];
         if (q > 0) {
-          float2 t = tw[k * q * tws];
+          V t = tw[k * q * tws];
           x = INV ? cmulc(x, t) : cmul(x, t);
         }
         v[b * R + q] = x;
@@ -177,7 +200,7 @@ __device__ __forceinline__ void stockham_stage(float2* buf, int n, int Ns, int n
     if (w < total) {
       int l, j;
       split_item<LINE_FAST>(w, nlines, nR, l, j);
-      float2* dst = buf + l * ls;
+      V* dst = buf + l * ls;
       const int k = j % Ns;
       const int d = (j / Ns) * Ns * R + k;
 #pragma unroll
@@ -188,24 +211,24 @@ __device__ __forceinline__ void stockham_stage(float2* buf, int n, int Ns, int n
 }
 
 // Direct DFT of every line (any n); requires ceil(nlines * n / kThreads) <= E.
-template <bool INV, bool LINE_FAST, int E>
-__device__ __forceinline__ void direct_dft(float2* buf, int n, int nlines, int es, int ls,
-                                           const float2* __restrict__ tw) {
+template <bool INV, bool LINE_FAST, int E, typename V>
+__device__ __forceinline__ void direct_dft(V* buf, int n, int nlines, int es, int ls,
+                                           const V* __restrict__ tw) {
   const int tid = threadIdx.x;
   const int total = nlines * n;
-  float2 v[E];
+  V v[E];
 #pragma unroll
   for (int b = 0; b < E; ++b) {
     const int w = tid + b * kThreads;
     if (w < total) {
       int l, k;
       split_item<LINE_FAST>(w, nlines, n, l, k);
-      const float2* src = buf + l * ls;
-      float2 acc = make_float2(0.f, 0.f);
+      const V* src = buf + l * ls;
+      V acc = CT<V>::mk(0, 0);
       int idx = 0;
       for (int j = 0; j < n; ++j) {
-        float2 x = src[j * es];
-        float2 t = tw[idx];
+        V x = src[j * es];
+        V t = tw[idx];
         acc = cadd(acc, INV ? cmulc(x, t) : cmul(x, t));
         idx += k;
         if (idx >= n) idx -= n;
@@ -227,21 +250,22 @@ __device__ __forceinline__ void direct_dft(float2* buf, int n, int nlines, int e
 }
 
 // Transform all lines in place.  Must be called by every thread of the block.
-template <bool INV, bool LINE_FAST, int E>
-__device__ void fft_lines(float2* buf, const AxisPlan& p, int nlines, int es, int ls) {
+template <bool INV, bool LINE_FAST, int E, typename V>
+__device__ __forceinline__ void fft_lines(V* buf, const AxisPlan& p, int nlines, int es, int ls) {
+  const V* tw = CT<V>::tw(p);
   if (p.nstages == 0) {
-    direct_dft<INV, LINE_FAST, E>(buf, p.n, nlines, es, ls, p.tw);
+    direct_dft<INV, LINE_FAST, E>(buf, p.n, nlines, es, ls, tw);
     return;
   }
   int Ns = 1;
   for (int s = 0; s < p.nstages; ++s) {
     const int R = p.radix[s];
     switch (R) {
-      case 8: stockham_stage<8, INV, LINE_FAST, E>(buf, p.n, Ns, nlines, es, ls, p.tw); break;
-      case 4: stockham_stage<4, INV, LINE_FAST, E>(buf, p.n, Ns, nlines, es, ls, p.tw); break;
-      case 2: stockham_stage<2, INV, LINE_FAST, E>(buf, p.n, Ns, nlines, es, ls, p.tw); break;
-      case 3: stockham_stage<3, INV, LINE_FAST, E>(buf, p.n, Ns, nlines, es, ls, p.tw); break;
-      default: stockham_stage<5, INV, LINE_FAST, E>(buf, p.n, Ns, nlines, es, ls, p.tw); break;
+      case 8: stockham_stage<8, INV, LINE_FAST, E>(buf, p.n, Ns, nlines, es, ls, tw); break;
+      case 4: stockham_stage<4, INV, LINE_FAST, E>(buf, p.n, Ns, nlines, es, ls, tw); break;
+      case 2: stockham_stage<2, INV, LINE_FAST, E>(buf, p.n, Ns, nlines, es, ls, tw); break;
+      case 3: stockham_stage<3, INV, LINE_FAST, E>(buf, p.n, Ns, nlines, es, ls, tw); break;
+      default: stockham_stage<5, INV, LINE_FAST, E>(buf, p.n, Ns, nlines, es, ls, tw); break;
     }
     Ns *= R;
   }

@@ -1,15 +1,32 @@
 // HIP kernels of the Lippmann-Schwinger iteration for gfx950.
 //
-// Data layout in HBM (one solver instance, R right-hand sides = load cases):
+// Formulation.  The strain is held as eps = base + delta.  The FP32 iteration
+// transforms only the correction's stress C delta; the divergence of the base
+// stress, d_base = xi~ . FFT(C base + tau), is evaluated in FP64 once per base
+// and stored.  Because Gamma0 sigma_hat depends on sigma_hat only through the
+// divergence xi~ . sigma_hat, the iteration
+//     d = d_base + xi~ . FFT32(C delta);  R = ||d|| / ||sigma_hat(0)||;
+//     delta += IFFT32(Gamma0(d))
+// is the reference iteration eps <- eps - Gamma0 C eps
+// (src/jaxmaterials/solver/lippmann_schwinger.py:208-246,
+//  cuda/src/lippmann_schwinger.cu:192-216) with the large stress of the base
+// kept out of the FP32 FFTs.  FP32 FFT rounding scales with the norm of the
+// transformed field; transforming C eps directly leaves a floor near 1e-6 of
+// the initial residual at 256^3, transforming C delta does not.  When the
+// residual has dropped by a fixed factor the host moves delta into the base
+// (k_rebase) and re-evaluates d_base (k_base_z, k_ypass<double2>, k_base_x).
+//
+// Data layout in HBM (one solver instance, R load cases):
 //   lambda, mu : [nmat][N] fp32, C order (x slowest, z fastest)
-//   eps        : [R][6][N] fp32, Voigt (xx,yy,zz,xy,xz,yz), tensor shear
-//   tau        : [R][6][N] fp32, optional polarisation added to the stress
+//   base, delta: [R][6][N] fp32, Voigt (xx,yy,zz,xy,xz,yz), tensor shear
+//   tau        : [R][6][N] fp32, optional polarisation (part of the base stress)
 //   spec       : [R][nx][ny][P][6] complex fp32, P = nz/2+1 (z half spectrum),
 //                the six Voigt components of one mode adjacent (48 B)
-// One iteration of the plain scheme is four passes over the spectrum:
-//   k_zline   : per (x,y) line: C2R along z -> eps += r -> Hooke -> R2C along z
+//   dbase      : [R][nx][ny*P][3] complex fp32, divergence of the base stress
+// One FP32 iteration is four passes over the spectrum:
+//   k_zline   : per (x,y) line: C2R along z -> delta += r -> Hooke -> R2C along z
 //   k_ypass   : forward FFT along y
-//   k_xpass   : forward FFT along x -> residual + Gamma0 -> inverse FFT along x
+//   k_xpass   : forward FFT along x -> divergence, residual, Gamma0 -> inverse FFT along x
 //   k_ypass   : inverse FFT along y
 // followed by k_check (deterministic FP64 residual reduction, stop test).
 #pragma once
@@ -44,53 +61,52 @@ struct RhsDesc {
 };
 
 struct RhsState {
-  double R0;    // normaliser of the relative tolerance
-  double R;     // latest residual
-  double den;   // ||sigma_hat(0)||_V of the latest check
-  int checks;   // residual checks done
-  int active;   // 1 while the load case iterates
+  double R0;      // normaliser of the relative tolerance
+  double R;       // latest residual
+  double den;     // ||sigma_hat(0)||_V of the latest check
+  double Rbase;   // residual when the current base was formed
+  double dcb[6];  // sigma_hat(0) of the base stress (FP64)
+  int checks;     // residual checks done
+  int active;     // 1 while the load case iterates
   int converged;
-  int status;   // 0 ok, 1 non-finite residual
+  int status;  // 0 ok, 1 non-finite residual
 };
 
 // z-line pass modes
 enum ZMode : int {
-  Z_INIT = 0,  // eps = eps_bar (written), sigma = C eps (+tau), R2C
-  Z_FWD = 1,   // eps read, sigma = C eps (+tau), R2C
-  Z_UPD = 2,   // C2R -> eps += r -> sigma -> R2C
-  Z_SET = 3,   // C2R -> eps = eps_bar - r -> sigma -> R2C   (end of a projection)
-  Z_PROJ = 4,  // eps read, sigma = C0 (eps - eps_bar), R2C  (start of a projection)
-  Z_AAINV = 5, // C2R -> r; history slot <- r, g = eps + r; FP64 Gram-row partials (no R2C)
-  Z_AAMIX = 6, // eps = sum_s alpha_s g^(s) -> sigma -> R2C
+  Z_UPD = 2,    // C2R -> delta += r -> sigma = C delta -> R2C
+  Z_SET = 3,    // C2R -> eps = eps_bar - r (end of a projection; no forward transform)
+  Z_PROJ = 4,   // eps read, sigma = C0 (eps - eps_bar), R2C  (start of a projection)
+  Z_AAINV = 5,  // C2R -> r; history slot <- r, g = delta + r; FP64 Gram-row partials (no R2C)
+  Z_AAMIX = 6,  // delta = sum_s alpha_s g^(s) -> sigma = C delta -> R2C
 };
 
 // Anderson history (lippmann_schwinger.py:177-231, doc/linear_elasticity.tex:335-370),
-// stored per load case as ring buffers of g^(s) = eps^(s) + r^(s) and r^(s).
+// stored per load case as ring buffers of g^(s) = delta^(s) + r^(s) and r^(s).
 struct AAParams {
-  float* G;              // [R][m1][6][N]
-  float* Rh;             // [R][m1][6][N]
-  const double* alpha;   // [R][m1] mixing weights by slot
-  double* dots;          // [R][nblk][m1] Gram-row partials by age
-  const int* head;       // [R] slot of the newest entry for this step
-  const int* filled;     // [R] number of filled slots including the newest
-  int m1;                // depth + 1
+  float* G;             // [R][m1][6][N]
+  float* Rh;            // [R][m1][6][N]
+  const double* alpha;  // [R][m1] mixing weights by slot
+  double* dots;         // [R][nblk][m1] Gram-row partials by age
+  const int* head;      // [R] slot of the newest entry for this step
+  const int* filled;    // [R] number of filled slots including the newest
+  int m1;               // depth + 1
 };
 
 __device__ __forceinline__ float4 ld4(const float2* p) { return *reinterpret_cast<const float4*>(p); }
-__device__ __forceinline__ void st4(float2* p, float4 v) { *reinterpret_cast<float4*>(p) = v; }
 
 // Hooke's law, isotropic: sigma_i = 2 mu eps_i + lambda tr(eps) [i<3]
 // (src/jaxmaterials/solver/lippmann_schwinger.py:20-33).
-__device__ __forceinline__ void hooke(const float* e, float lam, float mu, float* s) {
-  const float tr = e[0] + e[1] + e[2];
-  const float m2 = 2.f * mu;
+template <typename T>
+__device__ __forceinline__ void hooke(const T* e, T lam, T mu, T* s) {
+  const T tr = e[0] + e[1] + e[2];
+  const T m2 = T(2) * mu;
 #pragma unroll
-  for (int c = 0; c < 3; ++c) s[c] = fmaf(m2, e[c], lam * tr);
+  for (int c = 0; c < 3; ++c) s[c] = m2 * e[c] + lam * tr;
 #pragma unroll
   for (int c = 3; c < 6; ++c) s[c] = m2 * e[c];
 }
 
-// ---------------------------------------------------------------------------
 // wave64 / block reductions in FP64 with a fixed order (deterministic)
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -111,6 +127,26 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
   return s;  // valid in thread 0
 }
 
+// Split paired forward transforms z_p = a + i b (LDS lines) into the two half
+// spectra and store them at S[line][k][2p], S[line][k][2p+1].
+template <typename V>
+__device__ __forceinline__ void store_half_pairs(const V* lds, int L, int nz, int P, int ls, V* S,
+                                                 long long line0) {
+  using Sc = typename CT<V>::S;
+  const int items = L * P * 3;
+  for (int e = threadIdx.x; e < items; e += kThreads) {
+    const int p = e % 3;
+    const int k = (e / 3) % P;
+    const int l = e / (3 * P);
+    const V* Z = lds + (l * 3 + p) * ls;
+    const V zk = Z[k];
+    const V zm = Z[k == 0 ? 0 : nz - k];
+    V* o = S + ((line0 + l) * P + k) * 6 + 2 * p;
+    o[0] = CT<V>::mk(Sc(0.5) * (zk.x + zm.x), Sc(0.5) * (zk.y - zm.y));
+    o[1] = CT<V>::mk(Sc(0.5) * (zk.y + zm.y), Sc(-0.5) * (zk.x - zm.x));
+  }
+}
+
 // ---------------------------------------------------------------------------
 // z-line pass.  Block = Lz consecutive (x,y) lines of one load case.
 // LDS: Z[l][p][k], p = component pair (0,1),(2,3),(4,5): z_p = s_{2p} + i s_{2p+1}.
@@ -118,8 +154,7 @@ template <int MODE, int E>
 __global__ void __launch_bounds__(kThreads) k_zline(GridDev g, const RhsDesc* __restrict__ rhs,
                                                     RhsState* __restrict__ st, const float* __restrict__ lam,
                                                     const float* __restrict__ mu, float* __restrict__ eps,
-                                                    const float* __restrict__ tau, float2* __restrict__ spec,
-                                                    int Lz, AAParams aa, int write_eps) {
+                                                    float2* __restrict__ spec, int Lz, AAParams aa) {
   extern __shared__ float2 lds[];
   __shared__ double red[kThreads / 64];
   const int r = blockIdx.y;
@@ -132,7 +167,6 @@ __global__ void __launch_bounds__(kThreads) k_zline(GridDev g, const RhsDesc* __
   const long long N = g.N;
   float2* S = spec + (long long)r * g.M * 6;
   float* E_ = eps + (long long)r * 6 * N;
-  const float* T_ = tau ? tau + (long long)r * 6 * N : nullptr;
   const float* LA = lam + (long long)d.mat * N;
   const float* MU = mu + (long long)d.mat * N;
   const int ls = nz;
@@ -149,7 +183,10 @@ __global__ void __launch_bounds__(kThreads) k_zline(GridDev g, const RhsDesc* __
       float4 ab = ld4(S + ((long long)(line0 + l) * P + k) * 6 + 2 * p);
       float2 A = make_float2(ab.x, ab.y), B = make_float2(ab.z, ab.w);
       const bool self = (k == 0) || (2 * k == nz);
-      if (self) { A.y = 0.f; B.y = 0.f; }
+      if (self) {
+        A.y = 0.f;
+        B.y = 0.f;
+      }
       float2* Z = lds + (l * 3 + p) * ls;
       Z[k] = make_float2(A.x - B.y, A.y + B.x);
       if (!self) Z[nz - k] = make_float2(A.x + B.y, B.x - A.y);
@@ -159,7 +196,7 @@ __global__ void __launch_bounds__(kThreads) k_zline(GridDev g, const RhsDesc* __
   }
 
   if constexpr (MODE == Z_AAINV) {
-    // store r and g = eps + r in the newest slot; Gram-row partials by age
+    // store r and g = delta + r in the newest slot; Gram-row partials by age
     const int m1 = aa.m1;
     const int h = aa.head[r];
     const int f = aa.filled[r];
@@ -218,13 +255,7 @@ __global__ void __launch_bounds__(kThreads) k_zline(GridDev g, const RhsDesc* __
       const int l = e / nz, j = e - l * nz;
       const long long v = (long long)(line0 + l) * nz + j;
       float ev[6];
-      if constexpr (MODE == Z_INIT) {
-#pragma unroll
-        for (int c = 0; c < 6; ++c) {
-          ev[c] = d.eps_bar[c];
-          if (write_eps) E_[c * N + v] = ev[c];
-        }
-      } else if constexpr (MODE == Z_FWD || MODE == Z_PROJ) {
+      if constexpr (MODE == Z_PROJ) {
 #pragma unroll
         for (int c = 0; c < 6; ++c) ev[c] = E_[c * N + v];
       } else if constexpr (MODE == Z_AAMIX) {
@@ -264,65 +295,52 @@ __global__ void __launch_bounds__(kThreads) k_zline(GridDev g, const RhsDesc* __
 #pragma unroll
         for (int c = 0; c < 6; ++c) E_[c * N + v] = ev[c];
       }
-      float s[6];
-      if constexpr (MODE == Z_PROJ) {
-        float de[6];
+      if constexpr (MODE != Z_SET) {
+        float s[6];
+        if constexpr (MODE == Z_PROJ) {
+          float de[6];
 #pragma unroll
-        for (int c = 0; c < 6; ++c) de[c] = ev[c] - d.eps_bar[c];
-        hooke(de, d.lam0, d.mu0, s);
-      } else {
-        hooke(ev, LA[v], MU[v], s);
-        if (T_) {
-#pragma unroll
-          for (int c = 0; c < 6; ++c) s[c] += T_[c * N + v];
+          for (int c = 0; c < 6; ++c) de[c] = ev[c] - d.eps_bar[c];
+          hooke(de, d.lam0, d.mu0, s);
+        } else {
+          hooke(ev, LA[v], MU[v], s);
         }
-      }
 #pragma unroll
-      for (int p = 0; p < 3; ++p) lds[(l * 3 + p) * ls + j] = make_float2(s[2 * p], s[2 * p + 1]);
+        for (int p = 0; p < 3; ++p) lds[(l * 3 + p) * ls + j] = make_float2(s[2 * p], s[2 * p + 1]);
+      }
     }
   }
-  __syncthreads();
-  fft_lines<false, false, E>(lds, g.pz, 3 * L, 1, ls);
-
-  // split the paired transforms into two half spectra each and store
-  {
-    const int items = L * P * 3;
-    for (int e = tid; e < items; e += kThreads) {
-      const int p = e % 3;
-      const int k = (e / 3) % P;
-      const int l = e / (3 * P);
-      const float2* Z = lds + (l * 3 + p) * ls;
-      const float2 zk = Z[k];
-      const float2 zm = Z[k == 0 ? 0 : nz - k];
-      float4 ab;
-      ab.x = 0.5f * (zk.x + zm.x);
-      ab.y = 0.5f * (zk.y - zm.y);
-      ab.z = 0.5f * (zk.y + zm.y);
-      ab.w = -0.5f * (zk.x - zm.x);
-      st4(S + ((long long)(line0 + l) * P + k) * 6 + 2 * p, ab);
-    }
+  if constexpr (MODE == Z_SET) {
+    return;
+  } else {
+    __syncthreads();
+    fft_lines<false, false, E>(lds, g.pz, 3 * L, 1, ls);
+    __syncthreads();
+    store_half_pairs(lds, L, nz, P, ls, S, (long long)line0);
   }
 }
 
 // ---------------------------------------------------------------------------
 // y pass: FFT along y of T consecutive (kz, component) columns at fixed x.
-template <bool INV, int E>
+// rsel >= 0: the FP64 base evaluation (spec holds that one load case only).
+template <bool INV, int E, typename V>
 __global__ void __launch_bounds__(kThreads) k_ypass(GridDev g, const RhsState* __restrict__ st,
-                                                    float2* __restrict__ spec, int T, int ntiles) {
-  extern __shared__ float2 lds[];
-  const int r = blockIdx.y;
-  if (!st[r].active) return;
+                                                    V* __restrict__ spec, int T, int ntiles, int rsel) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  V* lds = reinterpret_cast<V*>(lds_raw);
+  const int r = rsel >= 0 ? rsel : (int)blockIdx.y;
+  if (rsel < 0 && !st[r].active) return;
   const int ny = g.ny;
   const int Q = 6 * g.P;  // columns per (x, y)
   const int x = blockIdx.x / ntiles;
   const int q0 = (blockIdx.x - x * ntiles) * T;
-  float2* S = spec + (long long)r * g.M * 6 + (long long)x * ny * Q;
+  V* S = spec + (rsel >= 0 ? 0LL : (long long)r * g.M * 6) + (long long)x * ny * Q;
   const int tid = threadIdx.x;
   const int items = ny * T;
   for (int e = tid; e < items; e += kThreads) {
     const int y = e / T, t = e - y * T;
     const int q = q0 + t;
-    lds[e] = (q < Q) ? S[(long long)y * Q + q] : make_float2(0.f, 0.f);
+    lds[e] = (q < Q) ? S[(long long)y * Q + q] : CT<V>::mk(0, 0);
   }
   __syncthreads();
   fft_lines<INV, true, E>(lds, g.py, T, T, 1);
@@ -334,20 +352,33 @@ __global__ void __launch_bounds__(kThreads) k_ypass(GridDev g, const RhsState* _
 }
 
 // ---------------------------------------------------------------------------
-// x pass: forward FFT along x -> residual (FP64 partial per block), DC capture,
-// Gamma0 -> inverse FFT along x.  Block = Tm consecutive (ky,kz) modes of one
-// load case, all six components, full x extent.
+// xi~(k) from the per-axis tables (fourier.py:9-58 closed form)
+__device__ __forceinline__ void xi_tilde(const GridDev& g, int kx, int ky, int kz, float& k0, float& k1,
+                                         float& k2) {
+  const float sxk = g.sx[kx], cxk = g.cx[kx];
+  const float syk = g.sy[ky], cyk = g.cy[ky];
+  const float szk = g.sz[kz], czk = g.cz[kz];
+  k0 = g.gx * sxk * cyk * czk;
+  k1 = g.gy * cxk * syk * czk;
+  k2 = g.gz * cxk * cyk * szk;
+}
+
+// ---------------------------------------------------------------------------
+// x pass: forward FFT along x -> divergence d = d_base + sigma_hat . xi~,
+// residual (FP64 partial per block), DC capture, Gamma0(d) -> inverse FFT
+// along x.  Block = Tm consecutive (ky,kz) modes of one load case, all six
+// components, full x extent.
 // Residual (lippmann_schwinger.py:118-143, fourier.cu:140-265):
-//   sum_k w_kz |xi~(k) . sigma_hat(k)|^2, w = 1 for kz = 0 or kz = nz/2, else 2.
-// Gamma0 (fourier.py:115-186, doc/linear_elasticity.tex:137-226), with n = xi0:
-//   v = sigma_hat n,  rho = n . v,
+//   sum_k w_kz |d(k)|^2, w = 1 for kz = 0 or kz = nz/2, else 2.
+// Gamma0 (fourier.py:115-186, doc/linear_elasticity.tex:137-226) in terms of
+// the divergence: n = xi~/|xi~|, v = d/|xi~| (= sigma_hat n), rho = n . v,
 //   r_hat = cA (n0 v0, n1 v1, n2 v2, (n0 v1 + n1 v0)/2, (n0 v2 + n2 v0)/2,
 //               (n1 v2 + n2 v1)/2) + cB rho (n0^2, n1^2, n2^2, n0 n1, n0 n2, n1 n2)
 template <int E>
 __global__ void __launch_bounds__(kThreads) k_xpass(GridDev g, const RhsDesc* __restrict__ rhs,
                                                     const RhsState* __restrict__ st, float2* __restrict__ spec,
-                                                    double* __restrict__ partials, float* __restrict__ dc, int Tm,
-                                                    int do_gamma) {
+                                                    const float2* __restrict__ dbase, double* __restrict__ partials,
+                                                    float* __restrict__ dc, int Tm, int do_gamma) {
   extern __shared__ float2 lds[];
   __shared__ double red[kThreads / 64];
   const int r = blockIdx.y;
@@ -357,6 +388,7 @@ __global__ void __launch_bounds__(kThreads) k_xpass(GridDev g, const RhsDesc* __
   const int m0 = blockIdx.x * Tm;
   const int cols = 6 * Tm;
   float2* S = spec + (long long)r * g.M * 6;
+  const float2* DB = dbase ? dbase + (long long)r * g.M * 3 : nullptr;
   const RhsDesc d = rhs[r];
   const int tid = threadIdx.x;
   const long long colmax = nm * 6;
@@ -384,13 +416,9 @@ __global__ void __launch_bounds__(kThreads) k_xpass(GridDev g, const RhsDesc* __
 #pragma unroll
       for (int c = 0; c < 6; ++c) dc[r * 6 + c] = s[c].x;
     }
-    const float sxk = g.sx[x], cxk = g.cx[x];
-    const float syk = g.sy[ky], cyk = g.cy[ky];
-    const float szk = g.sz[kz], czk = g.cz[kz];
-    const float k0 = g.gx * sxk * cyk * czk;
-    const float k1 = g.gy * cxk * syk * czk;
-    const float k2 = g.gz * cxk * cyk * szk;
-    // d = sigma_hat . xi~
+    float k0, k1, k2;
+    xi_tilde(g, x, ky, kz, k0, k1, k2);
+    // d = d_base + sigma_hat . xi~
     float2 d0, d1, d2;
     d0.x = k0 * s[0].x + k1 * s[3].x + k2 * s[4].x;
     d0.y = k0 * s[0].y + k1 * s[3].y + k2 * s[4].y;
@@ -398,6 +426,16 @@ __global__ void __launch_bounds__(kThreads) k_xpass(GridDev g, const RhsDesc* __
     d1.y = k0 * s[3].y + k1 * s[1].y + k2 * s[5].y;
     d2.x = k0 * s[4].x + k1 * s[5].x + k2 * s[2].x;
     d2.y = k0 * s[4].y + k1 * s[5].y + k2 * s[2].y;
+    if (DB) {
+      const float2* b = DB + ((long long)x * nm + gm) * 3;
+      const float2 b0 = b[0], b1 = b[1], b2 = b[2];
+      d0.x += b0.x;
+      d0.y += b0.y;
+      d1.x += b1.x;
+      d1.y += b1.y;
+      d2.x += b2.x;
+      d2.y += b2.y;
+    }
     const double wk = (kz == 0 || 2 * kz == g.nz) ? 1.0 : 2.0;
     acc += wk * ((double)d0.x * d0.x + (double)d0.y * d0.y + (double)d1.x * d1.x + (double)d1.y * d1.y +
                  (double)d2.x * d2.x + (double)d2.y * d2.y);
@@ -441,12 +479,101 @@ __global__ void __launch_bounds__(kThreads) k_xpass(GridDev g, const RhsDesc* __
 }
 
 // ---------------------------------------------------------------------------
+// FP64 base evaluation of one load case r (spec64 holds that load case only).
+// z lines: sigma = C base + tau in FP64, R2C along z in FP64.
+template <int E>
+__global__ void __launch_bounds__(kThreads) k_base_z(GridDev g, const RhsDesc* __restrict__ rhs, int r,
+                                                     const float* __restrict__ lam, const float* __restrict__ mu,
+                                                     const float* __restrict__ base, const float* __restrict__ tau,
+                                                     double2* __restrict__ spec64, int Lz) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  double2* lds = reinterpret_cast<double2*>(lds_raw);
+  const int nz = g.nz, P = g.P;
+  const int nlines_xy = g.nx * g.ny;
+  const int line0 = blockIdx.x * Lz;
+  const int L = min(Lz, nlines_xy - line0);
+  const RhsDesc d = rhs[r];
+  const long long N = g.N;
+  const float* B_ = base + (long long)r * 6 * N;
+  const float* T_ = tau ? tau + (long long)r * 6 * N : nullptr;
+  const float* LA = lam + (long long)d.mat * N;
+  const float* MU = mu + (long long)d.mat * N;
+  const int ls = nz;
+  const int items = L * nz;
+  for (int e = threadIdx.x; e < items; e += kThreads) {
+    const int l = e / nz, j = e - l * nz;
+    const long long v = (long long)(line0 + l) * nz + j;
+    double ev[6], s[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) ev[c] = B_[c * N + v];
+    hooke(ev, (double)LA[v], (double)MU[v], s);
+    if (T_) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) s[c] += T_[c * N + v];
+    }
+#pragma unroll
+    for (int p = 0; p < 3; ++p) lds[(l * 3 + p) * ls + j] = make_double2(s[2 * p], s[2 * p + 1]);
+  }
+  __syncthreads();
+  fft_lines<false, false, E>(lds, g.pz, 3 * L, 1, ls);
+  __syncthreads();
+  store_half_pairs(lds, L, nz, P, ls, spec64, (long long)line0);
+}
+
+// x lines: forward FFT along x in FP64, d_base = sigma_hat . xi~ (the FP32
+// tables of the iteration, promoted, so both use the same xi~), base DC.
+template <int E>
+__global__ void __launch_bounds__(kThreads) k_base_x(GridDev g, int r, const double2* __restrict__ spec64,
+                                                     float2* __restrict__ dbase, RhsState* __restrict__ st, int Tm) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  double2* lds = reinterpret_cast<double2*>(lds_raw);
+  const int nx = g.nx, P = g.P;
+  const long long nm = (long long)g.ny * P;
+  const int m0 = blockIdx.x * Tm;
+  const int cols = 6 * Tm;
+  const long long colmax = nm * 6;
+  const int items = nx * cols;
+  for (int e = threadIdx.x; e < items; e += kThreads) {
+    const int x = e / cols, t = e - x * cols;
+    const long long q = (long long)m0 * 6 + t;
+    lds[e] = (q < colmax) ? spec64[(long long)x * colmax + q] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  fft_lines<false, true, E>(lds, g.px, cols, cols, 1);
+  float2* DB = dbase + (long long)r * g.M * 3;
+  const int nmodes = nx * Tm;
+  for (int e = threadIdx.x; e < nmodes; e += kThreads) {
+    const int x = e / Tm, m = e - x * Tm;
+    const long long gm = (long long)m0 + m;
+    if (gm >= nm) continue;
+    const int ky = (int)(gm / P), kz = (int)(gm - (long long)ky * P);
+    const double2* s = lds + x * cols + 6 * m;
+    if (x == 0 && gm == 0) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) st[r].dcb[c] = s[c].x;
+    }
+    float f0, f1, f2;
+    xi_tilde(g, x, ky, kz, f0, f1, f2);
+    const double k0 = f0, k1 = f1, k2 = f2;
+    float2* o = DB + ((long long)x * nm + gm) * 3;
+    o[0] = make_float2((float)(k0 * s[0].x + k1 * s[3].x + k2 * s[4].x),
+                       (float)(k0 * s[0].y + k1 * s[3].y + k2 * s[4].y));
+    o[1] = make_float2((float)(k0 * s[3].x + k1 * s[1].x + k2 * s[5].x),
+                       (float)(k0 * s[3].y + k1 * s[1].y + k2 * s[5].y));
+    o[2] = make_float2((float)(k0 * s[4].x + k1 * s[5].x + k2 * s[2].x),
+                       (float)(k0 * s[4].y + k1 * s[5].y + k2 * s[2].y));
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Stop test per load case: R = sqrt(sum partials) / ||sigma_hat(0)||_V
 // (lippmann_schwinger.py:118-143; stop rule of lippmann_schwinger.cu:205:
 // R < max(rtol R0, atol), R0 = residual of the first check unless preset).
+// mode 0: a new check; 1: record R only (no stop test); 2: re-evaluation of
+// the current check after a rebase (not counted as a new check).
 __global__ void __launch_bounds__(kThreads) k_check(RhsState* __restrict__ st, const double* __restrict__ partials,
                                                     const float* __restrict__ dc, int nparts, double rtol,
-                                                    double atol, int record_only) {
+                                                    double atol, int mode) {
   __shared__ double red[kThreads / 64];
   const int r = blockIdx.x;
   if (!st[r].active) return;
@@ -455,34 +582,79 @@ __global__ void __launch_bounds__(kThreads) k_check(RhsState* __restrict__ st, c
   for (int i = threadIdx.x; i < nparts; i += kThreads) a += p[i];
   const double s = block_sum(a, red);
   if (threadIdx.x == 0) {
-    const float* z = dc + r * 6;
-    const double den = sqrt((double)z[0] * z[0] + (double)z[1] * z[1] + (double)z[2] * z[2] +
-                            2.0 * ((double)z[3] * z[3] + (double)z[4] * z[4] + (double)z[5] * z[5]));
-    const double R = sqrt(s) / den;
     RhsState q = st[r];
+    const float* z = dc + r * 6;
+    double z6[6];
+    for (int c = 0; c < 6; ++c) z6[c] = q.dcb[c] + (double)z[c];
+    const double den = sqrt(z6[0] * z6[0] + z6[1] * z6[1] + z6[2] * z6[2] +
+                            2.0 * (z6[3] * z6[3] + z6[4] * z6[4] + z6[5] * z6[5]));
+    const double R = sqrt(s) / den;
     q.R = R;
     q.den = den;
     if (!(R == R) || R > 1e300) {
       q.status = 1;
       q.active = 0;
-    } else if (!record_only) {
-      if (q.checks == 0 && !(q.R0 > 0.0)) q.R0 = R;
+    } else if (mode != 1) {
+      if (q.checks == 0 && mode == 0) {
+        if (!(q.R0 > 0.0)) q.R0 = R;
+        q.Rbase = R;
+      }
+      if (mode == 2) q.Rbase = R;
       const double thr = fmax(rtol * q.R0, atol);
       if (R < thr) {
         q.converged = 1;
         q.active = 0;
       }
     }
-    q.checks += 1;
+    if (mode != 2) q.checks += 1;
     st[r] = q;
   }
 }
 
 // ---------------------------------------------------------------------------
+// Move the correction into the base: base += delta, delta = 0; the Anderson
+// history g^(s) = delta^(s) + r^(s) is re-expressed relative to the new base.
+__global__ void __launch_bounds__(kThreads) k_rebase(const RhsState* __restrict__ st, long long N,
+                                                     float* __restrict__ base, float* __restrict__ delta,
+                                                     float* __restrict__ G, const int* __restrict__ filled, int m1) {
+  const int r = blockIdx.y;
+  if (!st[r].active) return;
+  const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= 6 * N) return;
+  const long long off = (long long)r * 6 * N + i;
+  const float dl = delta[off];
+  base[off] += dl;
+  delta[off] = 0.f;
+  if (G) {
+    const int f = filled[r];
+    float* Gb = G + (long long)r * m1 * 6 * N;
+    for (int s = 0; s < m1; ++s)
+      if (s < f) Gb[(long long)s * 6 * N + i] -= dl;
+  }
+}
+
+// For the load cases with sel[r] != 0: base = src (or eps_bar if src is
+// null), delta = 0 (if given).
+__global__ void __launch_bounds__(kThreads) k_init_fields(const RhsDesc* __restrict__ rhs, const int* __restrict__ sel,
+                                                          long long N, float* __restrict__ base,
+                                                          float* __restrict__ delta, const float* __restrict__ src) {
+  const int r = blockIdx.y;
+  if (sel && !sel[r]) return;
+  const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= 6 * N) return;
+  const long long off = (long long)r * 6 * N + i;
+  const int c = (int)(i / N);
+  base[off] = src ? src[off] : rhs[r].eps_bar[c];
+  if (delta) delta[off] = 0.f;
+}
+
+// ---------------------------------------------------------------------------
+// eps = base + delta (optionally written back into delta's buffer),
 // sigma = C eps (+tau), optional store, FP64 per-block component sums.
 __global__ void __launch_bounds__(kThreads) k_stress(GridDev g, const RhsDesc* __restrict__ rhs,
                                                      const int* __restrict__ sel, const float* __restrict__ lam,
-                                                     const float* __restrict__ mu, const float* __restrict__ eps,
+                                                     const float* __restrict__ mu, const float* __restrict__ base,
+                                                     float* __restrict__ eps, int write_eps,
                                                      const float* __restrict__ tau, float* __restrict__ sigma,
                                                      double* __restrict__ part6, int chunk) {
   __shared__ double red[kThreads / 64];
@@ -490,7 +662,8 @@ __global__ void __launch_bounds__(kThreads) k_stress(GridDev g, const RhsDesc* _
   if (sel && !sel[r]) return;
   const RhsDesc d = rhs[r];
   const long long N = g.N;
-  const float* E_ = eps + (long long)r * 6 * N;
+  const float* B_ = base ? base + (long long)r * 6 * N : nullptr;
+  float* E_ = eps + (long long)r * 6 * N;
   const float* T_ = tau ? tau + (long long)r * 6 * N : nullptr;
   float* S_ = sigma ? sigma + (long long)r * 6 * N : nullptr;
   const float* LA = lam + (long long)d.mat * N;
@@ -501,7 +674,11 @@ __global__ void __launch_bounds__(kThreads) k_stress(GridDev g, const RhsDesc* _
   for (long long v = v0 + threadIdx.x; v < v1; v += kThreads) {
     float e[6], s[6];
 #pragma unroll
-    for (int c = 0; c < 6; ++c) e[c] = E_[c * N + v];
+    for (int c = 0; c < 6; ++c) e[c] = E_[c * N + v] + (B_ ? B_[c * N + v] : 0.f);
+    if (write_eps) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) E_[c * N + v] = e[c];
+    }
     hooke(e, LA[v], MU[v], s);
     if (T_) {
 #pragma unroll
@@ -613,9 +790,16 @@ __global__ void __launch_bounds__(kThreads) k_aa_solve(const RhsState* __restric
     int piv = c;
     for (int i = c + 1; i < f; ++i)
       if (fabs(M[i][c]) > fabs(M[piv][c])) piv = i;
-    if (!(fabs(M[piv][c]) > 1e-300)) { ok = false; break; }
+    if (!(fabs(M[piv][c]) > 1e-300)) {
+      ok = false;
+      break;
+    }
     if (piv != c)
-      for (int j = c; j <= f; ++j) { const double t = M[c][j]; M[c][j] = M[piv][j]; M[piv][j] = t; }
+      for (int j = c; j <= f; ++j) {
+        const double t = M[c][j];
+        M[c][j] = M[piv][j];
+        M[piv][j] = t;
+      }
     for (int i = c + 1; i < f; ++i) {
       const double q = M[i][c] / M[c][c];
       for (int j = c; j <= f; ++j) M[i][j] -= q * M[c][j];

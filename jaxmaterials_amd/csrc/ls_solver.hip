@@ -4,9 +4,11 @@
 // then runs a loop with five device syncs and two blocking copies per iteration
 // (cuda/src/lippmann_schwinger.cu:112-234).  Here one solver object holds the
 // FFT plans (per-axis twiddle and sin/cos tables), a z-half spectrum for all
-// load cases and small FP64 reduction buffers; the loop keeps every decision
-// on the device (per-load-case active flags) and synchronises with the host
-// only every few iterations.
+// load cases, the strain as base + correction, the FP64-evaluated divergence
+// of the base stress and small FP64 reduction buffers.  The loop keeps every
+// per-iteration decision on the device (per-load-case active flags) and meets
+// the host only every few iterations, where it also decides when to move the
+// correction into the base (see ls_kernels.hpp).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -22,14 +24,14 @@
 
 using namespace lsk;
 
-#define LS_HIP(expr)                                                                          \
-  do {                                                                                        \
-    hipError_t e_ = (expr);                                                                   \
-    if (e_ != hipSuccess) {                                                                   \
+#define LS_HIP(expr)                                                                            \
+  do {                                                                                          \
+    hipError_t e_ = (expr);                                                                     \
+    if (e_ != hipSuccess) {                                                                     \
       fprintf(stderr, "liblippmannschwinger: HIP error %s at %s:%d: %s\n", hipGetErrorName(e_), \
-              __FILE__, __LINE__, #expr);                                                     \
-      return -1;                                                                              \
-    }                                                                                         \
+              __FILE__, __LINE__, #expr);                                                       \
+      return -1;                                                                                \
+    }                                                                                           \
   } while (0)
 
 namespace {
@@ -40,6 +42,7 @@ struct HostAxis {
   int n = 0;
   std::vector<int> radix;  // empty => direct DFT
   float2* d_tw = nullptr;
+  double2* d_tw64 = nullptr;
   float* d_s = nullptr;
   float* d_c = nullptr;
 };
@@ -47,24 +50,33 @@ struct HostAxis {
 std::vector<int> factorise(int n) {
   std::vector<int> r;
   int m = n, e2 = 0;
-  while (m % 2 == 0) { m /= 2; ++e2; }
-  std::vector<int> p3, p5;
-  while (m % 3 == 0) { m /= 3; p3.push_back(3); }
-  while (m % 5 == 0) { m /= 5; p5.push_back(5); }
-  if (m != 1) return {};  // prime factor > 5: direct DFT
-  // powers of two as 8s and 4s, a single 2 only for n = 2 * odd
-  if (e2 == 1) r.push_back(2);
-  else {
-    int e = e2;
-    std::vector<int> t;
-    while (e >= 3 && !(e == 4)) { t.push_back(8); e -= 3; }
-    while (e >= 2) { t.push_back(4); e -= 2; }
-    if (e == 1) t.push_back(2);
-    r = t;
+  while (m % 2 == 0) {
+    m /= 2;
+    ++e2;
   }
+  std::vector<int> p3, p5;
+  while (m % 3 == 0) {
+    m /= 3;
+    p3.push_back(3);
+  }
+  while (m % 5 == 0) {
+    m /= 5;
+    p5.push_back(5);
+  }
+  if (m != 1) return {};  // prime factor > 5: direct DFT
+  // powers of two as 8s and 4s (16 = 4 x 4), a single 2 only for odd exponents left over
+  int e = e2;
+  while (e >= 3 && e != 4) {
+    r.push_back(8);
+    e -= 3;
+  }
+  while (e >= 2) {
+    r.push_back(4);
+    e -= 2;
+  }
+  if (e == 1) r.push_back(2);
   for (int v : p3) r.push_back(v);
   for (int v : p5) r.push_back(v);
-  if (n == 1) r.clear();
   return r;
 }
 
@@ -84,17 +96,22 @@ AxisPlan to_dev(const HostAxis& a) {
   p.nstages = (int)a.radix.size();
   for (int i = 0; i < kMaxStages; ++i) p.radix[i] = i < p.nstages ? a.radix[i] : 0;
   p.tw = a.d_tw;
+  p.tw64 = a.d_tw64;
   return p;
 }
 
-constexpr int kEs[4] = {8, 16, 32, 64};
-constexpr size_t kMaxLds = 64 * 1024;
+constexpr size_t kMaxLds = 60 * 1024;
 
 struct Tile {
   int lines = 0;  // Lz (z), T (y) or Tm (x)
   int E = 0;
   size_t lds = 0;
 };
+
+int env_int(const char* name, int dflt) {
+  const char* v = std::getenv(name);
+  return v ? std::atoi(v) : dflt;
+}
 
 }  // namespace
 
@@ -107,14 +124,20 @@ struct ls_solver {
   bool own_stream = false;
   HostAxis ax[3];
   GridDev gd;
-  Tile tz, ty, tx;
+  Tile tz, ty, tx;        // FP32 iteration passes
+  Tile tz64, ty64, tx64;  // FP64 base evaluation passes
   int nblk_z = 0, nblk_y = 0, ntiles_y = 0, nblk_x = 0;
+  int nblk_z64 = 0, nblk_y64 = 0, ntiles_y64 = 0, nblk_x64 = 0;
   float2* spec = nullptr;
+  double2* spec64 = nullptr;  // one load case
+  float* base = nullptr;
+  float2* dbase = nullptr;
   double* partials = nullptr;
   float* dc = nullptr;
   RhsDesc* d_rhs = nullptr;
   RhsState* d_st = nullptr;
   RhsState* h_st = nullptr;  // pinned
+  int* d_sel = nullptr;
   double* part6 = nullptr;
   double* d_mean6 = nullptr;
   double* h_mean6 = nullptr;  // pinned
@@ -122,6 +145,8 @@ struct ls_solver {
   float* h_mm = nullptr;  // pinned
   int nblk_stress = 0, chunk_stress = 0;
   int nblk_mm = 0, chunk_mm = 0;
+  double rebase_factor = 1e-3;
+  int n_rebase = 0;
   // Anderson
   float* aaG = nullptr;
   float* aaR = nullptr;
@@ -130,8 +155,7 @@ struct ls_solver {
   double* aa_gram = nullptr;
   int* aa_head = nullptr;
   int* aa_filled = nullptr;
-  // timing of the last solve
-  hipEvent_t ev[2];
+  // timing of the last solve (LS_TIMING set): HIP events on the solver stream
   double t_ms[4] = {0, 0, 0, 0};
   int t_cnt[4] = {0, 0, 0, 0};
   bool timing = false;
@@ -149,13 +173,15 @@ int dalloc(T** p, size_t count) {
   return 0;
 }
 
-int build_axis(HostAxis& a, int n, double L) {
+int build_axis(HostAxis& a, int n) {
   a.n = n;
   a.radix = factorise(n);
   std::vector<float2> tw(n);
+  std::vector<double2> tw64(n);
   for (int t = 0; t < n; ++t) {
-    const double ang = -2.0 * M_PI * (double)t / (double)n;
-    tw[t] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+    const long double ang = -2.0L * 3.141592653589793238462643383279502884L * (long double)t / (long double)n;
+    tw64[t] = make_double2((double)cosl(ang), (double)sinl(ang));
+    tw[t] = make_float2((float)tw64[t].x, (float)tw64[t].y);
   }
   std::vector<float> s(n), c(n);
   for (int k = 0; k < n; ++k) {
@@ -164,20 +190,22 @@ int build_axis(HostAxis& a, int n, double L) {
   }
   s[0] = 0.f;
   if (n % 2 == 0) c[n / 2] = 0.f;
-  if (dalloc(&a.d_tw, n) || dalloc(&a.d_s, n) || dalloc(&a.d_c, n)) return -1;
+  if (dalloc(&a.d_tw, n) || dalloc(&a.d_tw64, n) || dalloc(&a.d_s, n) || dalloc(&a.d_c, n)) return -1;
   LS_HIP(hipMemcpy(a.d_tw, tw.data(), n * sizeof(float2), hipMemcpyHostToDevice));
+  LS_HIP(hipMemcpy(a.d_tw64, tw64.data(), n * sizeof(double2), hipMemcpyHostToDevice));
   LS_HIP(hipMemcpy(a.d_s, s.data(), n * sizeof(float), hipMemcpyHostToDevice));
   LS_HIP(hipMemcpy(a.d_c, c.data(), n * sizeof(float), hipMemcpyHostToDevice));
-  (void)L;
   return 0;
 }
 
-// Largest tile not above `want` lines whose FFT fits some E with LDS <= kMaxLds.
-// `lines_per_unit` FFT lines per tile unit; `granule` = elements per line.
-bool choose_tile(const HostAxis& a, int want, int lines_per_unit, size_t bytes_per_unit, Tile& t) {
+// Largest tile of at most `want` units whose FFT fits one of the register
+// budgets `Es` with LDS <= kMaxLds; a unit is `lines_per_unit` FFT lines of
+// `bytes_per_unit` LDS bytes.
+bool choose_tile(const HostAxis& a, int want, int lines_per_unit, size_t bytes_per_unit, const std::vector<int>& Es,
+                 Tile& t) {
   for (int u = std::max(1, want); u >= 1; --u) {
     if ((size_t)u * bytes_per_unit > kMaxLds) continue;
-    for (int E : kEs) {
+    for (int E : Es) {
       if (fits(a, u * lines_per_unit, E)) {
         t.lines = u;
         t.E = E;
@@ -189,36 +217,18 @@ bool choose_tile(const HostAxis& a, int want, int lines_per_unit, size_t bytes_p
   return false;
 }
 
-// ---- kernel launch dispatch on E ------------------------------------------
-template <int MODE>
-void launch_zline_E(ls_solver* s, int E, dim3 grid, const float* lam, const float* mu, float* eps,
-                    const float* tau, AAParams aa, int write_eps) {
-  const size_t lds = s->tz.lds;
-  switch (E) {
-#define LS_ZCASE(EV)                                                                                     \
-  case EV:                                                                                               \
-    k_zline<MODE, EV><<<grid, kThreads, lds, s->stream>>>(s->gd, s->d_rhs, s->d_st, lam, mu, eps, tau, \
-                                                          s->spec, s->tz.lines, aa, write_eps);          \
-    break;
-    LS_ZCASE(8)
-    LS_ZCASE(16)
-    LS_ZCASE(32)
-    LS_ZCASE(64)
-#undef LS_ZCASE
-  }
-}
-
 void mark(ls_solver* s, int kind) {
   if (!s->timing) return;
   hipEvent_t e;
-  hipEventCreate(&e);
-  hipEventRecord(e, s->stream);
+  if (hipEventCreate(&e) != hipSuccess) return;
+  (void)hipEventRecord(e, s->stream);
   s->tev.push_back(e);
   s->tkind.push_back(kind);
 }
 
+// ---- kernel launch dispatch on E ------------------------------------------
 template <int MODE>
-void zline(ls_solver* s, const float* lam, const float* mu, float* eps, const float* tau, int write_eps = 1) {
+void zline(ls_solver* s, const float* lam, const float* mu, float* eps) {
   AAParams aa;
   aa.G = s->aaG;
   aa.Rh = s->aaR;
@@ -228,7 +238,19 @@ void zline(ls_solver* s, const float* lam, const float* mu, float* eps, const fl
   aa.filled = s->aa_filled;
   aa.m1 = s->m1;
   dim3 grid(s->nblk_z, s->nrhs);
-  launch_zline_E<MODE>(s, s->tz.E, grid, lam, mu, eps, tau, aa, write_eps);
+  const size_t lds = s->tz.lds;
+  switch (s->tz.E) {
+#define LS_ZCASE(EV)                                                                                       \
+  case EV:                                                                                                 \
+    k_zline<MODE, EV><<<grid, kThreads, lds, s->stream>>>(s->gd, s->d_rhs, s->d_st, lam, mu, eps, s->spec, \
+                                                          s->tz.lines, aa);                                \
+    break;
+    LS_ZCASE(8)
+    LS_ZCASE(16)
+    LS_ZCASE(32)
+    LS_ZCASE(64)
+#undef LS_ZCASE
+  }
   mark(s, 0);
 }
 
@@ -236,14 +258,14 @@ void ypass(ls_solver* s, bool inv) {
   dim3 grid(s->nblk_y, s->nrhs);
   const size_t lds = s->ty.lds;
   switch (s->ty.E) {
-#define LS_YCASE(EV)                                                                                     \
-  case EV:                                                                                               \
-    if (inv)                                                                                             \
-      k_ypass<true, EV><<<grid, kThreads, lds, s->stream>>>(s->gd, s->d_st, s->spec, s->ty.lines,        \
-                                                             s->ntiles_y);                              \
-    else                                                                                                 \
-      k_ypass<false, EV><<<grid, kThreads, lds, s->stream>>>(s->gd, s->d_st, s->spec, s->ty.lines,       \
-                                                              s->ntiles_y);                             \
+#define LS_YCASE(EV)                                                                                      \
+  case EV:                                                                                                \
+    if (inv)                                                                                              \
+      k_ypass<true, EV, float2><<<grid, kThreads, lds, s->stream>>>(s->gd, s->d_st, s->spec, s->ty.lines, \
+                                                                     s->ntiles_y, -1);                   \
+    else                                                                                                  \
+      k_ypass<false, EV, float2><<<grid, kThreads, lds, s->stream>>>(s->gd, s->d_st, s->spec, s->ty.lines, \
+                                                                      s->ntiles_y, -1);                  \
     break;
     LS_YCASE(8)
     LS_YCASE(16)
@@ -254,14 +276,15 @@ void ypass(ls_solver* s, bool inv) {
   mark(s, 1);
 }
 
-void xpass(ls_solver* s, int do_gamma) {
+void xpass(ls_solver* s, int do_gamma, bool with_base = true) {
   dim3 grid(s->nblk_x, s->nrhs);
   const size_t lds = s->tx.lds;
+  const float2* db = with_base ? s->dbase : nullptr;
   switch (s->tx.E) {
-#define LS_XCASE(EV)                                                                                    \
-  case EV:                                                                                              \
-    k_xpass<EV><<<grid, kThreads, lds, s->stream>>>(s->gd, s->d_rhs, s->d_st, s->spec, s->partials,   \
-                                                     s->dc, s->tx.lines, do_gamma);                     \
+#define LS_XCASE(EV)                                                                                           \
+  case EV:                                                                                                     \
+    k_xpass<EV><<<grid, kThreads, lds, s->stream>>>(s->gd, s->d_rhs, s->d_st, s->spec, db, s->partials, s->dc, \
+                                                     s->tx.lines, do_gamma);                                   \
     break;
     LS_XCASE(8)
     LS_XCASE(16)
@@ -272,8 +295,47 @@ void xpass(ls_solver* s, int do_gamma) {
   mark(s, 2);
 }
 
-void check(ls_solver* s, double rtol, double atol, int record_only) {
-  k_check<<<s->nrhs, kThreads, 0, s->stream>>>(s->d_st, s->partials, s->dc, s->nblk_x, rtol, atol, record_only);
+void check(ls_solver* s, double rtol, double atol, int mode) {
+  k_check<<<s->nrhs, kThreads, 0, s->stream>>>(s->d_st, s->partials, s->dc, s->nblk_x, rtol, atol, mode);
+  mark(s, 3);
+}
+
+// FP64 evaluation of d_base for load case r from s->base (+tau).
+void base_eval_one(ls_solver* s, int r, const float* lam, const float* mu, const float* tau) {
+  switch (s->tz64.E) {
+#define LS_BZ(EV)                                                                                              \
+  case EV:                                                                                                     \
+    k_base_z<EV><<<s->nblk_z64, kThreads, s->tz64.lds, s->stream>>>(s->gd, s->d_rhs, r, lam, mu, s->base, tau, \
+                                                                    s->spec64, s->tz64.lines);                 \
+    break;
+    LS_BZ(8)
+    LS_BZ(16)
+    LS_BZ(32)
+#undef LS_BZ
+  }
+  switch (s->ty64.E) {
+#define LS_BY(EV)                                                                                              \
+  case EV:                                                                                                     \
+    k_ypass<false, EV, double2><<<s->nblk_y64, kThreads, s->ty64.lds, s->stream>>>(s->gd, s->d_st, s->spec64, \
+                                                                                    s->ty64.lines,           \
+                                                                                    s->ntiles_y64, r);       \
+    break;
+    LS_BY(8)
+    LS_BY(16)
+    LS_BY(32)
+#undef LS_BY
+  }
+  switch (s->tx64.E) {
+#define LS_BX(EV)                                                                                           \
+  case EV:                                                                                                  \
+    k_base_x<EV><<<s->nblk_x64, kThreads, s->tx64.lds, s->stream>>>(s->gd, r, s->spec64, s->dbase, s->d_st, \
+                                                                    s->tx64.lines);                         \
+    break;
+    LS_BX(8)
+    LS_BX(16)
+    LS_BX(32)
+#undef LS_BX
+  }
   mark(s, 3);
 }
 
@@ -288,39 +350,55 @@ int put_state(ls_solver* s) {
   return 0;
 }
 
+int put_sel(ls_solver* s, const std::vector<int>& sel) {
+  LS_HIP(hipMemcpyAsync(s->d_sel, sel.data(), s->nrhs * sizeof(int), hipMemcpyHostToDevice, s->stream));
+  LS_HIP(hipStreamSynchronize(s->stream));  // sel is a host temporary
+  return 0;
+}
+
+// Evaluate d_base for every load case active in the host state copy (which
+// must be current).
+void base_eval_active(ls_solver* s, const float* lam, const float* mu, const float* tau) {
+  for (int r = 0; r < s->nrhs; ++r)
+    if (s->h_st[r].active) base_eval_one(s, r, lam, mu, tau);
+}
+
+int zero_spec(ls_solver* s, const std::vector<int>& which) {
+  for (int r = 0; r < s->nrhs; ++r)
+    if (which[r])
+      LS_HIP(hipMemsetAsync(s->spec + (size_t)r * s->M * 6, 0, (size_t)s->M * 6 * sizeof(float2), s->stream));
+  return 0;
+}
+
 int any_active(const ls_solver* s) {
   for (int r = 0; r < s->nrhs; ++r)
     if (s->h_st[r].active) return 1;
   return 0;
 }
 
+std::vector<int> active_mask(const ls_solver* s) {
+  std::vector<int> m(s->nrhs);
+  for (int r = 0; r < s->nrhs; ++r) m[r] = s->h_st[r].active ? 1 : 0;
+  return m;
+}
+
 void free_solver(ls_solver* s) {
   if (!s) return;
   for (auto& a : s->ax) {
-    hipFree(a.d_tw);
-    hipFree(a.d_s);
-    hipFree(a.d_c);
+    (void)hipFree(a.d_tw);
+    (void)hipFree(a.d_tw64);
+    (void)hipFree(a.d_s);
+    (void)hipFree(a.d_c);
   }
-  hipFree(s->spec);
-  hipFree(s->partials);
-  hipFree(s->dc);
-  hipFree(s->d_rhs);
-  hipFree(s->d_st);
-  hipHostFree(s->h_st);
-  hipFree(s->part6);
-  hipFree(s->d_mean6);
-  hipHostFree(s->h_mean6);
-  hipFree(s->d_mm);
-  hipHostFree(s->h_mm);
-  hipFree(s->aaG);
-  hipFree(s->aaR);
-  hipFree(s->aa_alpha);
-  hipFree(s->aa_dots);
-  hipFree(s->aa_gram);
-  hipFree(s->aa_head);
-  hipFree(s->aa_filled);
-  for (auto e : s->tev) hipEventDestroy(e);
-  if (s->own_stream && s->stream) hipStreamDestroy(s->stream);
+  void* dev[] = {s->spec,    s->spec64,  s->base,    s->dbase,    s->partials, s->dc,       s->d_rhs,
+                 s->d_st,    s->d_sel,   s->part6,   s->d_mean6,  s->d_mm,     s->aaG,      s->aaR,
+                 s->aa_alpha, s->aa_dots, s->aa_gram, s->aa_head, s->aa_filled};
+  for (void* p : dev) (void)hipFree(p);
+  (void)hipHostFree(s->h_st);
+  (void)hipHostFree(s->h_mean6);
+  (void)hipHostFree(s->h_mm);
+  for (auto e : s->tev) (void)hipEventDestroy(e);
+  if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
 }
 
@@ -351,21 +429,30 @@ int create_impl(ls_solver* s, const int* voxels, const float* extents, int nrhs,
   s->nmat = nmat;
   s->depth = depth;
   s->m1 = depth + 1;
+  if (const char* rb = std::getenv("LS_REBASE_FACTOR")) s->rebase_factor = std::atof(rb);
   if (stream) {
     s->stream = (hipStream_t)stream;
   } else {
     LS_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     s->own_stream = true;
   }
-  const double L[3] = {s->Lx, s->Ly, s->Lz};
   const int n[3] = {s->nx, s->ny, s->nz};
   for (int d = 0; d < 3; ++d)
-    if (build_axis(s->ax[d], n[d], L[d])) return -1;
-  // tiles: aim for ~3-6k complex elements per block
-  if (!choose_tile(s->ax[2], std::max(1, 3072 / (3 * s->nz)), 3, (size_t)3 * s->nz * sizeof(float2), s->tz) ||
-      !choose_tile(s->ax[1], std::max(1, std::min(6 * s->P, 4096 / s->ny)), 1, (size_t)s->ny * sizeof(float2),
-                   s->ty) ||
-      !choose_tile(s->ax[0], std::max(1, 6144 / (6 * s->nx)), 6, (size_t)6 * s->nx * sizeof(float2), s->tx)) {
+    if (build_axis(s->ax[d], n[d])) return -1;
+  // tiles: aim for ~3-6k complex elements per block (overridable for tuning)
+  const std::vector<int> E32 = {8, 16, 32, 64}, E64 = {8, 16, 32};
+  const int wz = env_int("LS_TILE_Z", std::max(1, 3072 / (3 * s->nz)));
+  const int wy = env_int("LS_TILE_Y", std::max(1, std::min(6 * s->P, 4096 / s->ny)));
+  const int wx = env_int("LS_TILE_X", std::max(1, 6144 / (6 * s->nx)));
+  const bool ok32 = choose_tile(s->ax[2], wz, 3, (size_t)3 * s->nz * sizeof(float2), E32, s->tz) &&
+                    choose_tile(s->ax[1], wy, 1, (size_t)s->ny * sizeof(float2), E32, s->ty) &&
+                    choose_tile(s->ax[0], wx, 6, (size_t)6 * s->nx * sizeof(float2), E32, s->tx);
+  const bool ok64 =
+      choose_tile(s->ax[2], std::max(1, 2048 / (3 * s->nz)), 3, (size_t)3 * s->nz * sizeof(double2), E64, s->tz64) &&
+      choose_tile(s->ax[1], std::max(1, std::min(6 * s->P, 2048 / s->ny)), 1, (size_t)s->ny * sizeof(double2), E64,
+                  s->ty64) &&
+      choose_tile(s->ax[0], std::max(1, 2048 / (6 * s->nx)), 6, (size_t)6 * s->nx * sizeof(double2), E64, s->tx64);
+  if (!ok32 || !ok64) {
     fprintf(stderr, "liblippmannschwinger: no FFT tile fits for grid %d x %d x %d\n", s->nx, s->ny, s->nz);
     return -1;
   }
@@ -373,6 +460,10 @@ int create_impl(ls_solver* s, const int* voxels, const float* extents, int nrhs,
   s->ntiles_y = (6 * s->P + s->ty.lines - 1) / s->ty.lines;
   s->nblk_y = s->nx * s->ntiles_y;
   s->nblk_x = (int)(((long long)s->ny * s->P + s->tx.lines - 1) / s->tx.lines);
+  s->nblk_z64 = (int)((s->nx * (long long)s->ny + s->tz64.lines - 1) / s->tz64.lines);
+  s->ntiles_y64 = (6 * s->P + s->ty64.lines - 1) / s->ty64.lines;
+  s->nblk_y64 = s->nx * s->ntiles_y64;
+  s->nblk_x64 = (int)(((long long)s->ny * s->P + s->tx64.lines - 1) / s->tx64.lines);
   GridDev& g = s->gd;
   g.nx = s->nx;
   g.ny = s->ny;
@@ -392,8 +483,10 @@ int create_impl(ls_solver* s, const int* voxels, const float* extents, int nrhs,
   g.px = to_dev(s->ax[0]);
   g.py = to_dev(s->ax[1]);
   g.pz = to_dev(s->ax[2]);
-  if (dalloc(&s->spec, (size_t)nrhs * s->M * 6) || dalloc(&s->partials, (size_t)nrhs * s->nblk_x) ||
-      dalloc(&s->dc, (size_t)nrhs * 6) || dalloc(&s->d_rhs, nrhs) || dalloc(&s->d_st, nrhs))
+  if (dalloc(&s->spec, (size_t)nrhs * s->M * 6) || dalloc(&s->spec64, (size_t)s->M * 6) ||
+      dalloc(&s->base, (size_t)nrhs * 6 * s->N) || dalloc(&s->dbase, (size_t)nrhs * s->M * 3) ||
+      dalloc(&s->partials, (size_t)nrhs * s->nblk_x) || dalloc(&s->dc, (size_t)nrhs * 6) ||
+      dalloc(&s->d_rhs, nrhs) || dalloc(&s->d_st, nrhs) || dalloc(&s->d_sel, nrhs))
     return -1;
   LS_HIP(hipHostMalloc((void**)&s->h_st, nrhs * sizeof(RhsState), hipHostMallocDefault));
   s->chunk_stress = 8192;
@@ -427,29 +520,48 @@ int reset_aa(ls_solver* s) {
   return 0;
 }
 
-// One residual evaluation after a z-line forward pass: y fwd, x (residual [+ Gamma0]), check.
-void evaluate(ls_solver* s, double rtol, double atol, int record_only) {
-  ypass(s, false);
-  xpass(s, 1);
-  check(s, rtol, atol, record_only);
+void init_fields(ls_solver* s, const int* d_sel, float* delta, const float* src) {
+  dim3 grid((unsigned)((6 * s->N + kThreads - 1) / kThreads), s->nrhs);
+  k_init_fields<<<grid, kThreads, 0, s->stream>>>(s->d_rhs, d_sel, s->N, s->base, delta, src);
 }
 
-// One iteration for all active load cases (the spectrum holds Gamma0 sigma_hat
-// after the x pass of the previous check).
-void iterate(ls_solver* s, const float* lam, const float* mu, float* eps, const float* tau, double rtol,
-             double atol, bool evaluate_after) {
+// One residual evaluation after a z-line forward pass: y fwd, x (residual + Gamma0), check.
+void evaluate(ls_solver* s, double rtol, double atol, int mode) {
+  ypass(s, false);
+  xpass(s, 1);
+  check(s, rtol, atol, mode);
+}
+
+// One iteration for all active load cases (the spectrum holds Gamma0(d) after
+// the x pass of the previous check).
+void iterate(ls_solver* s, const float* lam, const float* mu, float* eps, double rtol, double atol,
+             bool evaluate_after) {
   ypass(s, true);
   if (s->depth == 0) {
-    zline<Z_UPD>(s, lam, mu, eps, tau);
+    zline<Z_UPD>(s, lam, mu, eps);
   } else {
     k_aa_advance<<<(s->nrhs + 63) / 64, 64, 0, s->stream>>>(s->d_st, s->aa_head, s->aa_filled, s->nrhs, s->m1);
-    zline<Z_AAINV>(s, lam, mu, eps, tau);
+    zline<Z_AAINV>(s, lam, mu, eps);
     k_aa_solve<<<s->nrhs, kThreads, 0, s->stream>>>(s->d_st, s->aa_dots, s->nblk_z, s->m1, s->aa_gram,
                                                     s->aa_alpha, s->aa_head, s->aa_filled);
     mark(s, 3);
-    zline<Z_AAMIX>(s, lam, mu, eps, tau);
+    zline<Z_AAMIX>(s, lam, mu, eps);
   }
   if (evaluate_after) evaluate(s, rtol, atol, 0);
+}
+
+// Move the corrections of all active load cases into their bases and
+// re-evaluate (host state copy must be current).
+int rebase(ls_solver* s, const float* lam, const float* mu, const float* tau, float* eps, double rtol,
+           double atol) {
+  dim3 grid((unsigned)((6 * s->N + kThreads - 1) / kThreads), s->nrhs);
+  k_rebase<<<grid, kThreads, 0, s->stream>>>(s->d_st, s->N, s->base, eps, s->aaG, s->aa_filled, s->m1);
+  base_eval_active(s, lam, mu, tau);
+  if (zero_spec(s, active_mask(s))) return -1;
+  xpass(s, 1);
+  check(s, rtol, atol, 2);
+  ++s->n_rebase;
+  return 0;
 }
 
 int solve_impl(ls_solver* s, const float* d_lambda, const float* d_mu, const int* rhs_mat, const float* eps_bar,
@@ -460,9 +572,10 @@ int solve_impl(ls_solver* s, const float* d_lambda, const float* d_mu, const int
     fprintf(stderr, "liblippmannschwinger: maxiter must be >= 0\n");
     return -1;
   }
-  for (auto e : s->tev) hipEventDestroy(e);
+  for (auto e : s->tev) (void)hipEventDestroy(e);
   s->tev.clear();
   s->tkind.clear();
+  s->n_rebase = 0;
   // reference medium per material: mid-range of the fields (lippmann_schwinger.cu:172-173)
   {
     dim3 grid(s->nblk_mm, s->nmat);
@@ -510,44 +623,49 @@ int solve_impl(ls_solver* s, const float* d_lambda, const float* d_mu, const int
   LS_HIP(hipMemcpyAsync(s->d_rhs, desc.data(), s->nrhs * sizeof(RhsDesc), hipMemcpyHostToDevice, s->stream));
   for (int r = 0; r < s->nrhs; ++r) {
     RhsState& q = s->h_st[r];
+    memset(&q, 0, sizeof(q));
     q.R0 = rel_ref > 0.0 ? rel_ref : 0.0;
-    q.R = 0.0;
-    q.den = 0.0;
-    q.checks = 0;
     q.active = 1;
-    q.converged = 0;
-    q.status = 0;
   }
   if (put_state(s)) return -1;
   if (reset_aa(s)) return -1;
   s->timing = (std::getenv("LS_TIMING") != nullptr);
+  mark(s, 3);
   const double drtol = rtol, datol = atol;
+  std::vector<int> all(s->nrhs, 1);
 
   // ---- start ----
   if (warm_start) {
-    // 1. residual of the uniform start, as the normaliser
+    // 1. residual of the uniform start, the normaliser of the tolerance
     if (!(rel_ref > 0.0)) {
-      zline<Z_INIT>(s, d_lambda, d_mu, d_eps, d_tau, 0);
-      ypass(s, false);
+      init_fields(s, nullptr, nullptr, nullptr);
+      base_eval_active(s, d_lambda, d_mu, d_tau);
+      if (zero_spec(s, all)) return -1;
       xpass(s, 0);
       check(s, drtol, datol, 1);
       if (sync_state(s)) return -1;
       for (int r = 0; r < s->nrhs; ++r) {
-        s->h_st[r].R0 = s->h_st[r].R;
-        s->h_st[r].checks = 0;
-        s->h_st[r].active = s->h_st[r].status == 0;
+        RhsState& q = s->h_st[r];
+        q.R0 = q.R;
+        q.checks = 0;
+        q.active = q.status == 0;
       }
       if (put_state(s)) return -1;
     }
-    // 2. project the start onto compatible fields with mean eps_bar
-    zline<Z_PROJ>(s, d_lambda, d_mu, d_eps, d_tau);
+    // 2. project the start onto compatible fields with mean eps_bar (FP32)
+    zline<Z_PROJ>(s, d_lambda, d_mu, d_eps);
     ypass(s, false);
-    xpass(s, 1);
+    xpass(s, 1, false);
     ypass(s, true);
-    zline<Z_SET>(s, d_lambda, d_mu, d_eps, d_tau);
-    evaluate(s, drtol, datol, 0);
+    zline<Z_SET>(s, d_lambda, d_mu, d_eps);
+    // 3. base = projected start, correction 0; evaluate
+    init_fields(s, nullptr, d_eps, d_eps);
+    base_eval_active(s, d_lambda, d_mu, d_tau);
+    if (zero_spec(s, all)) return -1;
+    xpass(s, 1);
+    check(s, drtol, datol, 0);
     if (sync_state(s)) return -1;
-    // 3. fall back to the uniform start where the warm start is worse
+    // 4. fall back to the uniform start where the warm start is worse
     std::vector<int> restart(s->nrhs), resume(s->nrhs);
     bool any_restart = false;
     for (int r = 0; r < s->nrhs; ++r) {
@@ -566,17 +684,23 @@ int solve_impl(ls_solver* s, const float* d_lambda, const float* d_mu, const int
           q.status = 0;
         }
       }
-      if (put_state(s)) return -1;
-      zline<Z_INIT>(s, d_lambda, d_mu, d_eps, d_tau, 1);
-      evaluate(s, drtol, datol, 0);
+      if (put_state(s) || put_sel(s, restart)) return -1;
+      init_fields(s, s->d_sel, d_eps, nullptr);
+      base_eval_active(s, d_lambda, d_mu, d_tau);
+      if (zero_spec(s, restart)) return -1;
+      xpass(s, 1);
+      check(s, drtol, datol, 0);
       if (sync_state(s)) return -1;
       for (int r = 0; r < s->nrhs; ++r)
         if (resume[r]) s->h_st[r].active = 1;
       if (put_state(s)) return -1;
     }
   } else {
-    zline<Z_INIT>(s, d_lambda, d_mu, d_eps, d_tau, 1);
-    evaluate(s, drtol, datol, 0);
+    init_fields(s, nullptr, d_eps, nullptr);
+    base_eval_active(s, d_lambda, d_mu, d_tau);
+    if (zero_spec(s, all)) return -1;
+    xpass(s, 1);
+    check(s, drtol, datol, 0);
   }
   LS_HIP(hipGetLastError());
 
@@ -589,7 +713,7 @@ int solve_impl(ls_solver* s, const float* d_lambda, const float* d_mu, const int
     printf("  iteration           ||r||   ||r||/||r_0||\n");
     printf("     %4d          %8.4e  %8.4e\n", 0, s->h_st[0].R, s->h_st[0].R / R0_print);
   }
-  const int sync_every = verbose == 2 ? 1 : (s->N >= (1LL << 21) ? 1 : 4);
+  const int sync_every = verbose == 2 ? 1 : env_int("LS_SYNC_EVERY", s->N >= (1LL << 21) ? 2 : 8);
   bool q7_done = false;
   std::vector<int> q7(s->nrhs, 0);
   for (int it = 1; it <= maxiter; ++it) {
@@ -599,49 +723,62 @@ int solve_impl(ls_solver* s, const float* d_lambda, const float* d_mu, const int
       if (!any_active(s)) break;
       if (verbose == 2 && it > 1)
         printf("     %4d          %8.4e  %8.4e\n", it - 1, s->h_st[0].R, s->h_st[0].R / R0_print);
+      // move the correction into the base once the residual has dropped enough
+      bool want = false;
+      for (int r = 0; r < s->nrhs; ++r) {
+        const RhsState& q = s->h_st[r];
+        if (q.active && q.R < s->rebase_factor * q.Rbase) want = true;
+      }
+      if (want && !last) {
+        if (rebase(s, d_lambda, d_mu, d_tau, d_eps, drtol, datol)) return -1;
+        if (sync_state(s)) return -1;
+        if (!any_active(s)) break;
+      }
     }
     if (last && legacy_q7 && d_sigma) {
       // reference quirk: when the loop runs out, the returned stress is the one
       // computed before the final update (lippmann_schwinger.cu:192-216, 230-231)
       for (int r = 0; r < s->nrhs; ++r) q7[r] = s->h_st[r].active ? 1 : 0;
-      int* d_sel = nullptr;
-      if (dalloc(&d_sel, s->nrhs)) return -1;
-      LS_HIP(hipMemcpyAsync(d_sel, q7.data(), s->nrhs * sizeof(int), hipMemcpyHostToDevice, s->stream));
+      if (put_sel(s, q7)) return -1;
       dim3 grid(s->nblk_stress, s->nrhs);
-      k_stress<<<grid, kThreads, 0, s->stream>>>(s->gd, s->d_rhs, d_sel, d_lambda, d_mu, d_eps, d_tau, d_sigma,
-                                                  s->part6, s->chunk_stress);
-      LS_HIP(hipStreamSynchronize(s->stream));
-      hipFree(d_sel);
+      k_stress<<<grid, kThreads, 0, s->stream>>>(s->gd, s->d_rhs, s->d_sel, d_lambda, d_mu, s->base, d_eps, 0,
+                                                  d_tau, d_sigma, s->part6, s->chunk_stress);
       q7_done = true;
     }
-    iterate(s, d_lambda, d_mu, d_eps, d_tau, drtol, datol, !last);
+    iterate(s, d_lambda, d_mu, d_eps, drtol, datol, !last);
   }
   LS_HIP(hipGetLastError());
   if (sync_state(s)) return -1;
 
-  // ---- outputs ----
-  std::vector<int> sel(s->nrhs, 1);
-  if (q7_done)
-    for (int r = 0; r < s->nrhs; ++r) sel[r] = q7[r] ? 0 : 1;
-  int* d_sel = nullptr;
-  if (dalloc(&d_sel, s->nrhs)) return -1;
-  LS_HIP(hipMemcpyAsync(d_sel, sel.data(), s->nrhs * sizeof(int), hipMemcpyHostToDevice, s->stream));
+  // ---- outputs: eps = base + delta, sigma = C eps + tau, FP64 means ----
   {
     dim3 grid(s->nblk_stress, s->nrhs);
-    k_stress<<<grid, kThreads, 0, s->stream>>>(s->gd, s->d_rhs, d_sel, d_lambda, d_mu, d_eps, d_tau, d_sigma,
-                                                s->part6, s->chunk_stress);
+    std::vector<int> sel(s->nrhs, 1);
+    if (q7_done)
+      for (int r = 0; r < s->nrhs; ++r) sel[r] = q7[r] ? 0 : 1;
+    if (put_sel(s, sel)) return -1;
+    k_stress<<<grid, kThreads, 0, s->stream>>>(s->gd, s->d_rhs, s->d_sel, d_lambda, d_mu, s->base, d_eps, 1, d_tau,
+                                                d_sigma, s->part6, s->chunk_stress);
+    if (q7_done) {
+      // strain of the load cases whose stress was saved before the last update
+      if (put_sel(s, q7)) return -1;
+      k_stress<<<grid, kThreads, 0, s->stream>>>(s->gd, s->d_rhs, s->d_sel, d_lambda, d_mu, s->base, d_eps, 1,
+                                                  d_tau, nullptr, nullptr, s->chunk_stress);
+    }
     k_sum6<<<s->nrhs, kThreads, 0, s->stream>>>(s->part6, s->nblk_stress, s->d_mean6, invN);
   }
   LS_HIP(hipGetLastError());
   LS_HIP(hipMemcpyAsync(s->h_mean6, s->d_mean6, s->nrhs * 6 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
   LS_HIP(hipStreamSynchronize(s->stream));
-  hipFree(d_sel);
 
   if (s->timing) {
-    for (int k = 0; k < 4; ++k) { s->t_ms[k] = 0.0; s->t_cnt[k] = 0; }
+    for (int k = 0; k < 4; ++k) {
+      s->t_ms[k] = 0.0;
+      s->t_cnt[k] = 0;
+    }
     for (size_t i = 1; i < s->tev.size(); ++i) {
       float ms = 0.f;
-      hipEventElapsedTime(&ms, s->tev[i - 1], s->tev[i]);
+      (void)hipEventElapsedTime(&ms, s->tev[i - 1], s->tev[i]);
       s->t_ms[s->tkind[i]] += ms;
       s->t_cnt[s->tkind[i]] += 1;
     }
@@ -701,7 +838,7 @@ int ls_solver_last_timing(ls_solver* s, double* ms4, int* count4) {
     if (ms4) ms4[k] = s->t_ms[k];
     if (count4) count4[k] = s->t_cnt[k];
   }
-  return 0;
+  return s->n_rebase;
 }
 
 int lippmann_schwinger_solve(float* lambda, float* mu, float* epsilon_bar, float* epsilon, float* sigma,
@@ -715,7 +852,8 @@ int lippmann_schwinger_solve(float* lambda, float* mu, float* epsilon_bar, float
   int it = -1;
   do {
     if (hipMalloc(&d_lam, N * sizeof(float)) != hipSuccess || hipMalloc(&d_mu, N * sizeof(float)) != hipSuccess ||
-        hipMalloc(&d_eps, 6 * N * sizeof(float)) != hipSuccess || hipMalloc(&d_sig, 6 * N * sizeof(float)) != hipSuccess) {
+        hipMalloc(&d_eps, 6 * N * sizeof(float)) != hipSuccess ||
+        hipMalloc(&d_sig, 6 * N * sizeof(float)) != hipSuccess) {
       fprintf(stderr, "liblippmannschwinger: device allocation failed\n");
       break;
     }
@@ -733,10 +871,10 @@ int lippmann_schwinger_solve(float* lambda, float* mu, float* epsilon_bar, float
     }
     it = iters;
   } while (0);
-  hipFree(d_lam);
-  hipFree(d_mu);
-  hipFree(d_eps);
-  hipFree(d_sig);
+  (void)hipFree(d_lam);
+  (void)hipFree(d_mu);
+  (void)hipFree(d_eps);
+  (void)hipFree(d_sig);
   ls_solver_destroy(s);
   return rc < 0 ? -1 : it;
 }
@@ -765,9 +903,9 @@ int lippmann_schwinger_effective_tensor(const float* lambda, const float* mu, co
     rc = solve_impl(s, d_lam, d_mu, mats, eb, nullptr, d_eps, nullptr, 0, rtol, atol, maxiter, 0.0, verbose, iters,
                     nullptr, ceff, false);
   } while (0);
-  hipFree(d_lam);
-  hipFree(d_mu);
-  hipFree(d_eps);
+  (void)hipFree(d_lam);
+  (void)hipFree(d_mu);
+  (void)hipFree(d_eps);
   ls_solver_destroy(s);
   return rc;
 }

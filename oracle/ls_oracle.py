@@ -89,6 +89,11 @@ class GridSpec:
         return (self.nx, self.ny, self.nz)
 
 
+def _shape(gs):
+    """(nx, ny, nz) of any grid-spec object with nx, ny, nz attributes."""
+    return (gs.nx, gs.ny, gs.nz)
+
+
 def compute_sigma(lmbda, mu, epsilon):
     """sigma_i = 2 mu eps_i + lambda tr(eps) [i<3]  (lippmann_schwinger.py:20-33)."""
     tr = epsilon[0] + epsilon[1] + epsilon[2]
@@ -178,7 +183,7 @@ def get_xizero(grid_spec, kz_half=False):
 
 def get_xizero_thresholded(grid_spec):
     """Literal restatement of fourier.py:9-85 (np.sin/np.cos, threshold 1e-12)."""
-    K = [2 * np.pi * np.arange(n) / n for n in grid_spec.shape]
+    K = [2 * np.pi * np.arange(n) / n for n in _shape(grid_spec)]
     X = np.meshgrid(*K, indexing="ij")
     xi = np.stack(
         [
@@ -273,7 +278,7 @@ def lippmann_schwinger_jax(lmbda, mu, epsilon_bar, grid_spec, rtol=1e-6, atol=1e
     mu0, lmbda0 = 0.5 * (mu.min() + mu.max()), 0.5 * (lmbda.min() + lmbda.max())
     xizero = get_xizero(grid_spec)
     xi = get_xi(grid_spec)
-    shp = (depth + 1, 6) + grid_spec.shape
+    shp = (depth + 1, 6) + _shape(grid_spec)
     epsilon = np.zeros(shp)
     epsilon[0] = np.asarray(epsilon_bar, dtype=np.float64)[:, None, None, None]
     residual = np.zeros(shp)
@@ -322,7 +327,7 @@ class _RealSpectral:
         return self.sfft.rfftn(f, axes=(-3, -2, -1), workers=self.workers)
 
     def inv(self, fh):
-        return self.sfft.irfftn(fh, s=self.gs.shape, axes=(-3, -2, -1), workers=self.workers)
+        return self.sfft.irfftn(fh, s=_shape(self.gs), axes=(-3, -2, -1), workers=self.workers)
 
     def residual(self, sigma_hat):
         return relative_divergence_rfft(sigma_hat, self.xi, self.gs.nz)
@@ -353,7 +358,7 @@ def lippmann_schwinger_rfft(lmbda, mu, epsilon_bar, grid_spec, rtol=1e-6, atol=0
     lmbda = np.asarray(lmbda, dtype=np.float64)
     mu = np.asarray(mu, dtype=np.float64)
     mu0, lmbda0 = 0.5 * (mu.min() + mu.max()), 0.5 * (lmbda.min() + lmbda.max())
-    shp = (depth + 1, 6) + grid_spec.shape
+    shp = (depth + 1, 6) + _shape(grid_spec)
     epsilon = np.zeros(shp)
     if epsilon0 is None:
         epsilon[0] = np.asarray(epsilon_bar, dtype=np.float64)[:, None, None, None]
@@ -416,7 +421,7 @@ def lippmann_schwinger_capi(lmbda, mu, epsilon_bar, grid_spec, rtol, atol, maxit
     # lippmann_schwinger.cu:172-173
     lmbda0 = 0.5 * (lmbda.max() + lmbda.min())
     mu0 = 0.5 * (mu.max() + mu.min())
-    eps = np.empty((6,) + grid_spec.shape)
+    eps = np.empty((6,) + _shape(grid_spec))
     eps[:] = np.asarray(epsilon_bar, dtype=np.float64)[:, None, None, None]
     sigma = None
     R0 = None
@@ -475,7 +480,7 @@ def material_properties(grid_spec, fibre_positions, fibre_radii, fibre_orientati
         indexing="ij",
     )
     nch = len(values_material)
-    data = np.zeros((nch,) + gs.shape)
+    data = np.zeros((nch,) + _shape(gs))
     for j in range(nch):
         vm = values_material[j]
         data[j] = vm

@@ -194,3 +194,26 @@ def test_rasteriser_matches_oracle(gpu):
         assert np.array_equal(lam.cpu().numpy(), lam_r)
         assert np.array_equal(mu.cpu().numpy(), mu_r)
         assert abs(avf - vf_r) < 1e-12
+
+
+@pytest.mark.parametrize("n", [64])
+def test_fp32_noise_floor_crossply(gpu, n):
+    """At rtol 1e-6 the FP32 iteration must track the FP64 oracle's iteration count
+    (+-1): the correction formulation keeps FP32 FFT rounding of the large base
+    stress out of the residual.  64^3 cross-ply, fibre radius 1.5, 4 layers."""
+    from jaxmaterials_amd.data import crossply_lame
+
+    shape = (n, n, n)
+    gs = GridSpec(*shape, float(n), float(n), float(n))
+    lam, mu, vf, nf = crossply_lame(shape, (float(n),) * 3, max(2, n // 16), 6.0 * n / 256, 0.5, seed=2)
+    s = LSSolver(gs, nrhs=1)
+    e0 = np.zeros((1, 6), np.float32)
+    e0[0, 0] = 1.0
+    out = s.solve(lam, mu, e0, rtol=1e-6, maxiter=1000)
+    lam_np, mu_np = lam.cpu().numpy().astype(np.float64), mu.cpu().numpy().astype(np.float64)
+    e_ref, _, it_ref, _ = O.lippmann_schwinger_rfft(lam_np, mu_np, e0[0].astype(np.float64), gs, rtol=1e-6,
+                                                    maxiter=1000, workers=8)
+    err = rel_l2(out["epsilon"][0].cpu().numpy(), e_ref)
+    print(f"crossply {n}^3: iter gpu={out['iters'][0]} oracle={it_ref} strain relL2={err:.2e}", flush=True)
+    assert abs(int(out["iters"][0]) - it_ref) <= 1
+    assert err <= 1e-5
