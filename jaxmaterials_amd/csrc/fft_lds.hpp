@@ -22,15 +22,49 @@
 namespace lsk {
 
 constexpr int kMaxStages = 16;
-constexpr int kThreads = 256;  // every FFT kernel runs 4 waves of 64
+constexpr int kThreads = 256;  // default block size (4 waves of 64)
+
+// Division by a runtime-invariant divisor d >= 1 for numerators 0 <= n < 2^31:
+// q = (umulhi(n, m) + n) >> s with s = ceil(log2 d),
+// m = floor(2^32 (2^s - d) / d) + 1 (Granlund & Montgomery 1994).  Integer
+// division by a runtime value costs tens of instructions on the GPU; the FFT
+// index arithmetic uses these instead.
+struct FastDiv {
+  unsigned d, m, s;
+};
+
+inline FastDiv make_fastdiv(unsigned d) {
+  FastDiv f;
+  f.d = d;
+  unsigned s = 0;
+  while ((1ull << s) < d) ++s;
+  f.s = s;
+  f.m = (unsigned)((((1ull << 32) * ((1ull << s) - d)) / d) + 1);
+  return f;
+}
+
+__device__ __forceinline__ int fdiv(int n, const FastDiv& f) {
+  return (int)((__umulhi((unsigned)n, f.m) + (unsigned)n) >> f.s);
+}
 
 struct AxisPlan {
   int n;
   int nstages;  // 0 => direct DFT
   int radix[kMaxStages];
+  FastDiv f_ns[kMaxStages];  // Ns = product of the earlier radices
+  FastDiv f_nr[kMaxStages];  // n / radix
+  FastDiv f_n;
   const float2* tw;    // n entries, exp(-2 pi i t / n)
   const double2* tw64;  // the same in FP64
 };
+
+// LDS index of element x of line l: lines of stride ls with element stride es,
+// or (PAD) contiguous lines with one pad slot per 16 elements, which spreads the
+// strided Stockham accesses over the LDS banks.
+template <bool PAD>
+__device__ __forceinline__ int lds_idx(int l, int x, int es, int ls) {
+  return PAD ? l * ls + x + (x >> 4) : l * ls + x * es;
+}
 
 template <typename V>
 struct CT;
@@ -152,38 +186,44 @@ __device__ __forceinline__ void dftR(V* v) {
 // walk across lines (use when lines are adjacent in LDS, i.e. ls == 1), else
 // along a line (use when elements are adjacent, es == 1).
 template <bool LINE_FAST>
-__device__ __forceinline__ void split_item(int w, int nlines, int per_line, int& l, int& j) {
+__device__ __forceinline__ void split_item(int w, const FastDiv& f_lines, const FastDiv& f_per_line, int& l,
+                                           int& j) {
   if (LINE_FAST) {
-    j = w / nlines;
-    l = w - j * nlines;
+    j = fdiv(w, f_lines);
+    l = w - j * (int)f_lines.d;
   } else {
-    l = w / per_line;
-    j = w - l * per_line;
+    l = fdiv(w, f_per_line);
+    j = w - l * (int)f_per_line.d;
   }
 }
 
-// One Stockham stage of radix R, in place in LDS.  E = register elements per
-// thread; the host guarantees ceil(nlines * n / R / kThreads) * R <= E.
-template <int R, bool INV, bool LINE_FAST, int E, typename V>
-__device__ __forceinline__ void stockham_stage(V* buf, int n, int Ns, int nlines, int es, int ls,
+// One Stockham stage of radix R, in place in LDS, by a block of NT threads.
+// E = register elements per thread; the host guarantees
+// ceil(nlines * n / R / NT) * R <= E.
+template <int R, bool INV, bool LINE_FAST, bool PAD, int E, int NT, typename V>
+__device__ __forceinline__ void stockham_stage(V* buf, int n, const FastDiv& f_ns, const FastDiv& f_nr,
+                                               const FastDiv& f_lines, int nlines, int es, int ls,
                                                const V* __restrict__ tw) {
   constexpr int NB = E / R;
   const int tid = threadIdx.x;
-  const int nR = n / R;
+  const int nR = (int)f_nr.d;
+  const int Ns = (int)f_ns.d;
   const int total = nlines * nR;
   const int tws = n / (Ns * R);
   V v[NB * R];
+  int dst0[NB];
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
-    const int w = tid + b * kThreads;
+    const int w = tid + b * NT;
+    dst0[b] = -1;
     if (w < total) {
       int l, j;
-      split_item<LINE_FAST>(w, nlines, nR, l, j);
-      const V* src = buf + l * ls;
-      const int k = j % Ns;
+      split_item<LINE_FAST>(w, f_lines, f_nr, l, j);
+      const int jq = fdiv(j, f_ns);
+      const int k = j - jq * Ns;
 #pragma unroll
       for (int q = 0; q < R; ++q) {
-        V x = src[(j + q * nR) * es];
+        V x = buf[lds_idx<PAD>(l, j + q * nR, es, ls)];
         if (q > 0) {
           V t = tw[k * q * tws];
           x = INV ? cmulc(x, t) : cmul(x, t);
@@ -191,43 +231,39 @@ __device__ __forceinline__ void stockham_stage(V* buf, int n, int Ns, int nlines
         v[b * R + q] = x;
       }
       dftR<R, INV>(&v[b * R]);
+      dst0[b] = l * 0x10000 + (jq * Ns * R + k);  // (line, first output index), both < 2^16
     }
   }
   __syncthreads();
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
-    const int w = tid + b * kThreads;
-    if (w < total) {
-      int l, j;
-      split_item<LINE_FAST>(w, nlines, nR, l, j);
-      V* dst = buf + l * ls;
-      const int k = j % Ns;
-      const int d = (j / Ns) * Ns * R + k;
+    if (dst0[b] >= 0) {
+      const int l = dst0[b] >> 16, d = dst0[b] & 0xffff;
 #pragma unroll
-      for (int q = 0; q < R; ++q) dst[(d + q * Ns) * es] = v[b * R + q];
+      for (int q = 0; q < R; ++q) buf[lds_idx<PAD>(l, d + q * Ns, es, ls)] = v[b * R + q];
     }
   }
   __syncthreads();
 }
 
-// Direct DFT of every line (any n); requires ceil(nlines * n / kThreads) <= E.
-template <bool INV, bool LINE_FAST, int E, typename V>
-__device__ __forceinline__ void direct_dft(V* buf, int n, int nlines, int es, int ls,
-                                           const V* __restrict__ tw) {
+// Direct DFT of every line (any n); requires ceil(nlines * n / NT) <= E.
+template <bool INV, bool LINE_FAST, bool PAD, int E, int NT, typename V>
+__device__ __forceinline__ void direct_dft(V* buf, const FastDiv& f_n, const FastDiv& f_lines, int nlines, int es,
+                                           int ls, const V* __restrict__ tw) {
   const int tid = threadIdx.x;
+  const int n = (int)f_n.d;
   const int total = nlines * n;
   V v[E];
 #pragma unroll
   for (int b = 0; b < E; ++b) {
-    const int w = tid + b * kThreads;
+    const int w = tid + b * NT;
     if (w < total) {
       int l, k;
-      split_item<LINE_FAST>(w, nlines, n, l, k);
-      const V* src = buf + l * ls;
+      split_item<LINE_FAST>(w, f_lines, f_n, l, k);
       V acc = CT<V>::mk(0, 0);
       int idx = 0;
       for (int j = 0; j < n; ++j) {
-        V x = src[j * es];
+        V x = buf[lds_idx<PAD>(l, j, es, ls)];
         V t = tw[idx];
         acc = cadd(acc, INV ? cmulc(x, t) : cmul(x, t));
         idx += k;
@@ -239,35 +275,39 @@ __device__ __forceinline__ void direct_dft(V* buf, int n, int nlines, int es, in
   __syncthreads();
 #pragma unroll
   for (int b = 0; b < E; ++b) {
-    const int w = tid + b * kThreads;
+    const int w = tid + b * NT;
     if (w < total) {
       int l, k;
-      split_item<LINE_FAST>(w, nlines, n, l, k);
-      buf[l * ls + k * es] = v[b];
+      split_item<LINE_FAST>(w, f_lines, f_n, l, k);
+      buf[lds_idx<PAD>(l, k, es, ls)] = v[b];
     }
   }
   __syncthreads();
 }
 
 // Transform all lines in place.  Must be called by every thread of the block.
-template <bool INV, bool LINE_FAST, int E, typename V>
-__device__ __forceinline__ void fft_lines(V* buf, const AxisPlan& p, int nlines, int es, int ls) {
+// f_lines (LINE_FAST layouts) divides by nlines; lines must be < 2^15 and
+// line lengths < 2^16.
+template <bool INV, bool LINE_FAST, bool PAD, int E, int NT, typename V>
+__device__ __forceinline__ void fft_lines(V* buf, const AxisPlan& p, const FastDiv& f_lines, int nlines, int es,
+                                          int ls) {
   const V* tw = CT<V>::tw(p);
   if (p.nstages == 0) {
-    direct_dft<INV, LINE_FAST, E>(buf, p.n, nlines, es, ls, tw);
+    direct_dft<INV, LINE_FAST, PAD, E, NT>(buf, p.f_n, f_lines, nlines, es, ls, tw);
     return;
   }
-  int Ns = 1;
   for (int s = 0; s < p.nstages; ++s) {
     const int R = p.radix[s];
+#define LS_STAGE(RV) \
+  stockham_stage<RV, INV, LINE_FAST, PAD, E, NT>(buf, p.n, p.f_ns[s], p.f_nr[s], f_lines, nlines, es, ls, tw)
     switch (R) {
-      case 8: stockham_stage<8, INV, LINE_FAST, E>(buf, p.n, Ns, nlines, es, ls, tw); break;
-      case 4: stockham_stage<4, INV, LINE_FAST, E>(buf, p.n, Ns, nlines, es, ls, tw); break;
-      case 2: stockham_stage<2, INV, LINE_FAST, E>(buf, p.n, Ns, nlines, es, ls, tw); break;
-      case 3: stockham_stage<3, INV, LINE_FAST, E>(buf, p.n, Ns, nlines, es, ls, tw); break;
-      default: stockham_stage<5, INV, LINE_FAST, E>(buf, p.n, Ns, nlines, es, ls, tw); break;
+      case 8: LS_STAGE(8); break;
+      case 4: LS_STAGE(4); break;
+      case 2: LS_STAGE(2); break;
+      case 3: LS_STAGE(3); break;
+      default: LS_STAGE(5); break;
     }
-    Ns *= R;
+#undef LS_STAGE
   }
 }
 

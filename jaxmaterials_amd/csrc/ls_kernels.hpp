@@ -51,6 +51,22 @@ struct GridDev {
   const float* sz;
   const float* cz;
   AxisPlan px, py, pz;
+  FastDiv f_nz, f_P;  // divisions by nz and P
+  int lsz;            // padded LDS line stride of the z-line kernels
+};
+
+// Layout of d_base matched to the tiles of the FP32 x pass: [tile][x][m][3]
+// with tile = gm / Tm and m = gm % Tm, so each x-pass block reads one
+// contiguous range.
+struct DbLayout {
+  FastDiv f_Tm;        // modes per tile of the FP32 x pass
+  int nx;
+  long long per_rhs;  // complex entries per load case
+  __device__ __forceinline__ long long index(int x, int gm) const {
+    const int t = fdiv(gm, f_Tm);
+    const int m = gm - t * (int)f_Tm.d;
+    return (((long long)t * nx + x) * (long long)f_Tm.d + m) * 3;
+  }
 };
 
 struct RhsDesc {
@@ -114,6 +130,7 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+template <int NT = kThreads>
 __device__ __forceinline__ double block_sum(double v, double* red) {
   v = wave_sum(v);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -122,25 +139,35 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
   __syncthreads();
   double s = 0.0;
   if (threadIdx.x == 0) {
-    for (int i = 0; i < kThreads / 64; ++i) s += red[i];
+    for (int i = 0; i < NT / 64; ++i) s += red[i];
   }
   return s;  // valid in thread 0
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
+// Remap the linear workgroup id so that every XCD works through a contiguous
+// range of tiles and neighbouring tiles (adjacent HBM segments) share an L2.
+__device__ __forceinline__ int xcd_remap(int b, int n) {
+  constexpr int X = 8;
+  const int per = n / X, rem = n % X;
+  const int x = b % X, k = b / X;
+  return x * per + min(x, rem) + k;
+}
+
 // Split paired forward transforms z_p = a + i b (LDS lines) into the two half
 // spectra and store them at S[line][k][2p], S[line][k][2p+1].
-template <typename V>
-__device__ __forceinline__ void store_half_pairs(const V* lds, int L, int nz, int P, int ls, V* S,
-                                                 long long line0) {
+template <int NT, typename V>
+__device__ __forceinline__ void store_half_pairs(const V* lds, const GridDev& g, int L, V* S, long long line0) {
   using Sc = typename CT<V>::S;
+  const int nz = g.nz, P = g.P, ls = g.lsz;
   const int items = L * P * 3;
-  for (int e = threadIdx.x; e < items; e += kThreads) {
-    const int p = e % 3;
-    const int k = (e / 3) % P;
-    const int l = e / (3 * P);
-    const V* Z = lds + (l * 3 + p) * ls;
-    const V zk = Z[k];
-    const V zm = Z[k == 0 ? 0 : nz - k];
+  for (int e = threadIdx.x; e < items; e += NT) {
+    const int e3 = e / 3;
+    const int p = e - 3 * e3;
+    const int l = fdiv(e3, g.f_P);
+    const int k = e3 - l * P;
+    const V zk = lds[lds_idx<true>(l * 3 + p, k, 1, ls)];
+    const V zm = lds[lds_idx<true>(l * 3 + p, k == 0 ? 0 : nz - k, 1, ls)];
     V* o = S + ((line0 + l) * P + k) * 6 + 2 * p;
     o[0] = CT<V>::mk(Sc(0.5) * (zk.x + zm.x), Sc(0.5) * (zk.y - zm.y));
     o[1] = CT<V>::mk(Sc(0.5) * (zk.y + zm.y), Sc(-0.5) * (zk.x - zm.x));
@@ -150,18 +177,20 @@ __device__ __forceinline__ void store_half_pairs(const V* lds, int L, int nz, in
 // ---------------------------------------------------------------------------
 // z-line pass.  Block = Lz consecutive (x,y) lines of one load case.
 // LDS: Z[l][p][k], p = component pair (0,1),(2,3),(4,5): z_p = s_{2p} + i s_{2p+1}.
-template <int MODE, int E>
-__global__ void __launch_bounds__(kThreads) k_zline(GridDev g, const RhsDesc* __restrict__ rhs,
-                                                    RhsState* __restrict__ st, const float* __restrict__ lam,
-                                                    const float* __restrict__ mu, float* __restrict__ eps,
-                                                    float2* __restrict__ spec, int Lz, AAParams aa) {
+template <int MODE, int E, int NT>
+__global__ void __launch_bounds__(NT) k_zline(GridDev g, const RhsDesc* __restrict__ rhs,
+                                              RhsState* __restrict__ st, const float* __restrict__ lam,
+                                              const float* __restrict__ mu, float* __restrict__ eps,
+                                              float2* __restrict__ spec, int Lz, int ntiles, AAParams aa) {
   extern __shared__ float2 lds[];
-  __shared__ double red[kThreads / 64];
-  const int r = blockIdx.y;
+  __shared__ double red[NT / 64];
+  const int lin = xcd_remap(blockIdx.x, gridDim.x);
+  const int r = lin / ntiles;
+  const int tile = lin - r * ntiles;
   if (!st[r].active) return;
   const int nz = g.nz, P = g.P;
   const int nlines_xy = g.nx * g.ny;
-  const int line0 = blockIdx.x * Lz;
+  const int line0 = tile * Lz;
   const int L = min(Lz, nlines_xy - line0);
   const RhsDesc d = rhs[r];
   const long long N = g.N;
@@ -169,17 +198,19 @@ __global__ void __launch_bounds__(kThreads) k_zline(GridDev g, const RhsDesc* __
   float* E_ = eps + (long long)r * 6 * N;
   const float* LA = lam + (long long)d.mat * N;
   const float* MU = mu + (long long)d.mat * N;
-  const int ls = nz;
+  const int ls = g.lsz;
   const int tid = threadIdx.x;
+  const FastDiv f_none = g.f_nz;  // unused by the line-contiguous layout
 
   if constexpr (MODE == Z_UPD || MODE == Z_SET || MODE == Z_AAINV) {
     // gather the half spectra into the paired full-length lines (C2R semantics:
     // imaginary parts of the k = 0 and k = nz/2 coefficients are ignored)
     const int items = L * P * 3;
-    for (int e = tid; e < items; e += kThreads) {
-      const int p = e % 3;
-      const int k = (e / 3) % P;
-      const int l = e / (3 * P);
+    for (int e = tid; e < items; e += NT) {
+      const int e3 = e / 3;
+      const int p = e - 3 * e3;
+      const int l = fdiv(e3, g.f_P);
+      const int k = e3 - l * P;
       float4 ab = ld4(S + ((long long)(line0 + l) * P + k) * 6 + 2 * p);
       float2 A = make_float2(ab.x, ab.y), B = make_float2(ab.z, ab.w);
       const bool self = (k == 0) || (2 * k == nz);
@@ -187,12 +218,11 @@ __global__ void __launch_bounds__(kThreads) k_zline(GridDev g, const RhsDesc* __
         A.y = 0.f;
         B.y = 0.f;
       }
-      float2* Z = lds + (l * 3 + p) * ls;
-      Z[k] = make_float2(A.x - B.y, A.y + B.x);
-      if (!self) Z[nz - k] = make_float2(A.x + B.y, B.x - A.y);
+      lds[lds_idx<true>(l * 3 + p, k, 1, ls)] = make_float2(A.x - B.y, A.y + B.x);
+      if (!self) lds[lds_idx<true>(l * 3 + p, nz - k, 1, ls)] = make_float2(A.x + B.y, B.x - A.y);
     }
     __syncthreads();
-    fft_lines<true, false, E>(lds, g.pz, 3 * L, 1, ls);
+    fft_lines<true, false, true, E, NT>(lds, g.pz, f_none, 3 * L, 1, ls);
   }
 
   if constexpr (MODE == Z_AAINV) {
@@ -207,13 +237,13 @@ __global__ void __launch_bounds__(kThreads) k_zline(GridDev g, const RhsDesc* __
 #pragma unroll
     for (int s = 0; s < kMaxDepth1; ++s) acc[s] = 0.0;
     const int items = L * nz;
-    for (int e = tid; e < items; e += kThreads) {
-      const int l = e / nz, j = e - l * nz;
+    for (int e = tid; e < items; e += NT) {
+      const int l = fdiv(e, g.f_nz), j = e - l * nz;
       const long long v = (long long)(line0 + l) * nz + j;
       float rr[6];
 #pragma unroll
       for (int p = 0; p < 3; ++p) {
-        const float2 z = lds[(l * 3 + p) * ls + j];
+        const float2 z = lds[lds_idx<true>(l * 3 + p, j, 1, ls)];
         rr[2 * p] = z.x;
         rr[2 * p + 1] = z.y;
       }
@@ -242,8 +272,8 @@ __global__ void __launch_bounds__(kThreads) k_zline(GridDev g, const RhsDesc* __
       }
     }
     for (int s = 0; s < m1; ++s) {
-      const double bs = block_sum(acc[s], red);
-      if (tid == 0) aa.dots[((long long)r * gridDim.x + blockIdx.x) * m1 + s] = (s < f) ? bs : 0.0;
+      const double bs = block_sum<NT>(acc[s], red);
+      if (tid == 0) aa.dots[((long long)r * ntiles + tile) * m1 + s] = (s < f) ? bs : 0.0;
     }
     return;
   }
@@ -251,8 +281,8 @@ __global__ void __launch_bounds__(kThreads) k_zline(GridDev g, const RhsDesc* __
   // pointwise: update / Hooke, write sigma pairs into LDS
   {
     const int items = L * nz;
-    for (int e = tid; e < items; e += kThreads) {
-      const int l = e / nz, j = e - l * nz;
+    for (int e = tid; e < items; e += NT) {
+      const int l = fdiv(e, g.f_nz), j = e - l * nz;
       const long long v = (long long)(line0 + l) * nz + j;
       float ev[6];
       if constexpr (MODE == Z_PROJ) {
@@ -281,7 +311,7 @@ __global__ void __launch_bounds__(kThreads) k_zline(GridDev g, const RhsDesc* __
         float rr[6];
 #pragma unroll
         for (int p = 0; p < 3; ++p) {
-          const float2 z = lds[(l * 3 + p) * ls + j];
+          const float2 z = lds[lds_idx<true>(l * 3 + p, j, 1, ls)];
           rr[2 * p] = z.x;
           rr[2 * p + 1] = z.y;
         }
@@ -306,7 +336,7 @@ __global__ void __launch_bounds__(kThreads) k_zline(GridDev g, const RhsDesc* __
           hooke(ev, LA[v], MU[v], s);
         }
 #pragma unroll
-        for (int p = 0; p < 3; ++p) lds[(l * 3 + p) * ls + j] = make_float2(s[2 * p], s[2 * p + 1]);
+        for (int p = 0; p < 3; ++p) lds[lds_idx<true>(l * 3 + p, j, 1, ls)] = make_float2(s[2 * p], s[2 * p + 1]);
       }
     }
   }
@@ -314,38 +344,42 @@ __global__ void __launch_bounds__(kThreads) k_zline(GridDev g, const RhsDesc* __
     return;
   } else {
     __syncthreads();
-    fft_lines<false, false, E>(lds, g.pz, 3 * L, 1, ls);
+    fft_lines<false, false, true, E, NT>(lds, g.pz, f_none, 3 * L, 1, ls);
     __syncthreads();
-    store_half_pairs(lds, L, nz, P, ls, S, (long long)line0);
+    store_half_pairs<NT>(lds, g, L, S, (long long)line0);
   }
 }
 
 // ---------------------------------------------------------------------------
 // y pass: FFT along y of T consecutive (kz, component) columns at fixed x.
 // rsel >= 0: the FP64 base evaluation (spec holds that one load case only).
-template <bool INV, int E, typename V>
-__global__ void __launch_bounds__(kThreads) k_ypass(GridDev g, const RhsState* __restrict__ st,
-                                                    V* __restrict__ spec, int T, int ntiles, int rsel) {
+template <bool INV, int E, int NT, typename V>
+__global__ void __launch_bounds__(NT) k_ypass(GridDev g, const RhsState* __restrict__ st, V* __restrict__ spec,
+                                              FastDiv f_T, int ntiles, int rsel) {
+  const int T = (int)f_T.d;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   V* lds = reinterpret_cast<V*>(lds_raw);
-  const int r = rsel >= 0 ? rsel : (int)blockIdx.y;
+  const int lin = xcd_remap(blockIdx.x, gridDim.x);
+  const int per_rhs = g.nx * ntiles;
+  const int r = rsel >= 0 ? rsel : lin / per_rhs;
+  const int tile = rsel >= 0 ? lin : lin - r * per_rhs;
   if (rsel < 0 && !st[r].active) return;
   const int ny = g.ny;
   const int Q = 6 * g.P;  // columns per (x, y)
-  const int x = blockIdx.x / ntiles;
-  const int q0 = (blockIdx.x - x * ntiles) * T;
+  const int x = tile / ntiles;
+  const int q0 = (tile - x * ntiles) * T;
   V* S = spec + (rsel >= 0 ? 0LL : (long long)r * g.M * 6) + (long long)x * ny * Q;
   const int tid = threadIdx.x;
   const int items = ny * T;
-  for (int e = tid; e < items; e += kThreads) {
-    const int y = e / T, t = e - y * T;
+  for (int e = tid; e < items; e += NT) {
+    const int y = fdiv(e, f_T), t = e - y * T;
     const int q = q0 + t;
     lds[e] = (q < Q) ? S[(long long)y * Q + q] : CT<V>::mk(0, 0);
   }
   __syncthreads();
-  fft_lines<INV, true, E>(lds, g.py, T, T, 1);
-  for (int e = tid; e < items; e += kThreads) {
-    const int y = e / T, t = e - y * T;
+  fft_lines<INV, true, false, E, NT>(lds, g.py, f_T, T, T, 1);
+  for (int e = tid; e < items; e += NT) {
+    const int y = fdiv(e, f_T), t = e - y * T;
     const int q = q0 + t;
     if (q < Q) S[(long long)y * Q + q] = lds[e];
   }
@@ -374,40 +408,44 @@ __device__ __forceinline__ void xi_tilde(const GridDev& g, int kx, int ky, int k
 // the divergence: n = xi~/|xi~|, v = d/|xi~| (= sigma_hat n), rho = n . v,
 //   r_hat = cA (n0 v0, n1 v1, n2 v2, (n0 v1 + n1 v0)/2, (n0 v2 + n2 v0)/2,
 //               (n1 v2 + n2 v1)/2) + cB rho (n0^2, n1^2, n2^2, n0 n1, n0 n2, n1 n2)
-template <int E>
-__global__ void __launch_bounds__(kThreads) k_xpass(GridDev g, const RhsDesc* __restrict__ rhs,
-                                                    const RhsState* __restrict__ st, float2* __restrict__ spec,
-                                                    const float2* __restrict__ dbase, double* __restrict__ partials,
-                                                    float* __restrict__ dc, int Tm, int do_gamma) {
+template <int E, int NT>
+__global__ void __launch_bounds__(NT) k_xpass(GridDev g, const RhsDesc* __restrict__ rhs,
+                                              const RhsState* __restrict__ st, float2* __restrict__ spec,
+                                              const float2* __restrict__ dbase, double* __restrict__ partials,
+                                              float* __restrict__ dc, FastDiv f_Tm, FastDiv f_cols, int ntiles,
+                                              DbLayout dl, int do_gamma) {
   extern __shared__ float2 lds[];
-  __shared__ double red[kThreads / 64];
-  const int r = blockIdx.y;
+  __shared__ double red[NT / 64];
+  const int Tm = (int)f_Tm.d;
+  const int lin = xcd_remap(blockIdx.x, gridDim.x);
+  const int r = lin / ntiles;
+  const int tile = lin - r * ntiles;
   if (!st[r].active) return;
   const int nx = g.nx, P = g.P;
   const long long nm = (long long)g.ny * P;  // modes per x plane
-  const int m0 = blockIdx.x * Tm;
+  const int m0 = tile * Tm;
   const int cols = 6 * Tm;
   float2* S = spec + (long long)r * g.M * 6;
-  const float2* DB = dbase ? dbase + (long long)r * g.M * 3 : nullptr;
+  const float2* DB = dbase ? dbase + (long long)r * dl.per_rhs + (long long)tile * nx * Tm * 3 : nullptr;
   const RhsDesc d = rhs[r];
   const int tid = threadIdx.x;
   const long long colmax = nm * 6;
   const int items = nx * cols;
-  for (int e = tid; e < items; e += kThreads) {
-    const int x = e / cols, t = e - x * cols;
+  for (int e = tid; e < items; e += NT) {
+    const int x = fdiv(e, f_cols), t = e - x * cols;
     const long long q = (long long)m0 * 6 + t;
     lds[e] = (q < colmax) ? S[(long long)x * colmax + q] : make_float2(0.f, 0.f);
   }
   __syncthreads();
-  fft_lines<false, true, E>(lds, g.px, cols, cols, 1);
+  fft_lines<false, true, false, E, NT>(lds, g.px, f_cols, cols, cols, 1);
 
   double acc = 0.0;
   const int nmodes = nx * Tm;
-  for (int e = tid; e < nmodes; e += kThreads) {
-    const int x = e / Tm, m = e - x * Tm;
-    const long long gm = (long long)m0 + m;
+  for (int e = tid; e < nmodes; e += NT) {
+    const int x = fdiv(e, f_Tm), m = e - x * Tm;
+    const int gm = m0 + m;
     if (gm >= nm) continue;
-    const int ky = (int)(gm / P), kz = (int)(gm - (long long)ky * P);
+    const int ky = fdiv(gm, g.f_P), kz = gm - ky * P;
     float2* t = lds + x * cols + 6 * m;
     float2 s[6];
 #pragma unroll
@@ -427,7 +465,7 @@ __global__ void __launch_bounds__(kThreads) k_xpass(GridDev g, const RhsDesc* __
     d2.x = k0 * s[4].x + k1 * s[5].x + k2 * s[2].x;
     d2.y = k0 * s[4].y + k1 * s[5].y + k2 * s[2].y;
     if (DB) {
-      const float2* b = DB + ((long long)x * nm + gm) * 3;
+      const float2* b = DB + (x * Tm + m) * 3;
       const float2 b0 = b[0], b1 = b[1], b2 = b[2];
       d0.x += b0.x;
       d0.y += b0.y;
@@ -466,13 +504,13 @@ __global__ void __launch_bounds__(kThreads) k_xpass(GridDev g, const RhsDesc* __
       for (int c = 0; c < 6; ++c) t[c] = o[c];
     }
   }
-  const double bs = block_sum(acc, red);
-  if (tid == 0) partials[(long long)r * gridDim.x + blockIdx.x] = bs;
+  const double bs = block_sum<NT>(acc, red);
+  if (tid == 0) partials[(long long)r * ntiles + tile] = bs;
   __syncthreads();
   if (!do_gamma) return;
-  fft_lines<true, true, E>(lds, g.px, cols, cols, 1);
-  for (int e = tid; e < items; e += kThreads) {
-    const int x = e / cols, t = e - x * cols;
+  fft_lines<true, true, false, E, NT>(lds, g.px, f_cols, cols, cols, 1);
+  for (int e = tid; e < items; e += NT) {
+    const int x = fdiv(e, f_cols), t = e - x * cols;
     const long long q = (long long)m0 * 6 + t;
     if (q < colmax) S[(long long)x * colmax + q] = lds[e];
   }
@@ -481,16 +519,16 @@ __global__ void __launch_bounds__(kThreads) k_xpass(GridDev g, const RhsDesc* __
 // ---------------------------------------------------------------------------
 // FP64 base evaluation of one load case r (spec64 holds that load case only).
 // z lines: sigma = C base + tau in FP64, R2C along z in FP64.
-template <int E>
-__global__ void __launch_bounds__(kThreads) k_base_z(GridDev g, const RhsDesc* __restrict__ rhs, int r,
-                                                     const float* __restrict__ lam, const float* __restrict__ mu,
-                                                     const float* __restrict__ base, const float* __restrict__ tau,
-                                                     double2* __restrict__ spec64, int Lz) {
+template <int E, int NT>
+__global__ void __launch_bounds__(NT) k_base_z(GridDev g, const RhsDesc* __restrict__ rhs, int r,
+                                               const float* __restrict__ lam, const float* __restrict__ mu,
+                                               const float* __restrict__ base, const float* __restrict__ tau,
+                                               double2* __restrict__ spec64, int Lz) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   double2* lds = reinterpret_cast<double2*>(lds_raw);
   const int nz = g.nz, P = g.P;
   const int nlines_xy = g.nx * g.ny;
-  const int line0 = blockIdx.x * Lz;
+  const int line0 = xcd_remap(blockIdx.x, gridDim.x) * Lz;
   const int L = min(Lz, nlines_xy - line0);
   const RhsDesc d = rhs[r];
   const long long N = g.N;
@@ -498,10 +536,10 @@ __global__ void __launch_bounds__(kThreads) k_base_z(GridDev g, const RhsDesc* _
   const float* T_ = tau ? tau + (long long)r * 6 * N : nullptr;
   const float* LA = lam + (long long)d.mat * N;
   const float* MU = mu + (long long)d.mat * N;
-  const int ls = nz;
+  const int ls = g.lsz;
   const int items = L * nz;
-  for (int e = threadIdx.x; e < items; e += kThreads) {
-    const int l = e / nz, j = e - l * nz;
+  for (int e = threadIdx.x; e < items; e += NT) {
+    const int l = fdiv(e, g.f_nz), j = e - l * nz;
     const long long v = (long long)(line0 + l) * nz + j;
     double ev[6], s[6];
 #pragma unroll
@@ -512,41 +550,43 @@ __global__ void __launch_bounds__(kThreads) k_base_z(GridDev g, const RhsDesc* _
       for (int c = 0; c < 6; ++c) s[c] += T_[c * N + v];
     }
 #pragma unroll
-    for (int p = 0; p < 3; ++p) lds[(l * 3 + p) * ls + j] = make_double2(s[2 * p], s[2 * p + 1]);
+    for (int p = 0; p < 3; ++p) lds[lds_idx<true>(l * 3 + p, j, 1, ls)] = make_double2(s[2 * p], s[2 * p + 1]);
   }
   __syncthreads();
-  fft_lines<false, false, E>(lds, g.pz, 3 * L, 1, ls);
+  fft_lines<false, false, true, E, NT>(lds, g.pz, g.f_nz, 3 * L, 1, ls);
   __syncthreads();
-  store_half_pairs(lds, L, nz, P, ls, spec64, (long long)line0);
+  store_half_pairs<NT>(lds, g, L, spec64, (long long)line0);
 }
 
 // x lines: forward FFT along x in FP64, d_base = sigma_hat . xi~ (the FP32
 // tables of the iteration, promoted, so both use the same xi~), base DC.
-template <int E>
-__global__ void __launch_bounds__(kThreads) k_base_x(GridDev g, int r, const double2* __restrict__ spec64,
-                                                     float2* __restrict__ dbase, RhsState* __restrict__ st, int Tm) {
+template <int E, int NT>
+__global__ void __launch_bounds__(NT) k_base_x(GridDev g, int r, const double2* __restrict__ spec64,
+                                               float2* __restrict__ dbase, RhsState* __restrict__ st, FastDiv f_Tm,
+                                               FastDiv f_cols, DbLayout dl) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
+  const int Tm = (int)f_Tm.d;
   double2* lds = reinterpret_cast<double2*>(lds_raw);
   const int nx = g.nx, P = g.P;
   const long long nm = (long long)g.ny * P;
-  const int m0 = blockIdx.x * Tm;
+  const int m0 = xcd_remap(blockIdx.x, gridDim.x) * Tm;
   const int cols = 6 * Tm;
   const long long colmax = nm * 6;
   const int items = nx * cols;
-  for (int e = threadIdx.x; e < items; e += kThreads) {
-    const int x = e / cols, t = e - x * cols;
+  for (int e = threadIdx.x; e < items; e += NT) {
+    const int x = fdiv(e, f_cols), t = e - x * cols;
     const long long q = (long long)m0 * 6 + t;
     lds[e] = (q < colmax) ? spec64[(long long)x * colmax + q] : make_double2(0.0, 0.0);
   }
   __syncthreads();
-  fft_lines<false, true, E>(lds, g.px, cols, cols, 1);
-  float2* DB = dbase + (long long)r * g.M * 3;
+  fft_lines<false, true, false, E, NT>(lds, g.px, f_cols, cols, cols, 1);
+  float2* DB = dbase + (long long)r * dl.per_rhs;
   const int nmodes = nx * Tm;
-  for (int e = threadIdx.x; e < nmodes; e += kThreads) {
-    const int x = e / Tm, m = e - x * Tm;
-    const long long gm = (long long)m0 + m;
+  for (int e = threadIdx.x; e < nmodes; e += NT) {
+    const int x = fdiv(e, f_Tm), m = e - x * Tm;
+    const int gm = m0 + m;
     if (gm >= nm) continue;
-    const int ky = (int)(gm / P), kz = (int)(gm - (long long)ky * P);
+    const int ky = fdiv(gm, g.f_P), kz = gm - ky * P;
     const double2* s = lds + x * cols + 6 * m;
     if (x == 0 && gm == 0) {
 #pragma unroll
@@ -555,7 +595,7 @@ __global__ void __launch_bounds__(kThreads) k_base_x(GridDev g, int r, const dou
     float f0, f1, f2;
     xi_tilde(g, x, ky, kz, f0, f1, f2);
     const double k0 = f0, k1 = f1, k2 = f2;
-    float2* o = DB + ((long long)x * nm + gm) * 3;
+    float2* o = DB + dl.index(x, gm);
     o[0] = make_float2((float)(k0 * s[0].x + k1 * s[3].x + k2 * s[4].x),
                        (float)(k0 * s[0].y + k1 * s[3].y + k2 * s[4].y));
     o[1] = make_float2((float)(k0 * s[3].x + k1 * s[1].x + k2 * s[5].x),

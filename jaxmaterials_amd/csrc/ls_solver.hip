@@ -80,12 +80,12 @@ std::vector<int> factorise(int n) {
   return r;
 }
 
-bool fits(const HostAxis& a, int nlines, int E) {
+bool fits(const HostAxis& a, int nlines, int E, int NT) {
   const long long tot = (long long)nlines * a.n;
-  if (a.radix.empty()) return (tot + kThreads - 1) / kThreads <= E;
+  if (a.radix.empty()) return (tot + NT - 1) / NT <= E;
   for (int R : a.radix) {
     const long long bf = tot / R;
-    if (((bf + kThreads - 1) / kThreads) * R > E) return false;
+    if (((bf + NT - 1) / NT) * R > E) return false;
   }
   return true;
 }
@@ -94,7 +94,14 @@ AxisPlan to_dev(const HostAxis& a) {
   AxisPlan p;
   p.n = a.n;
   p.nstages = (int)a.radix.size();
-  for (int i = 0; i < kMaxStages; ++i) p.radix[i] = i < p.nstages ? a.radix[i] : 0;
+  int ns = 1;
+  for (int i = 0; i < kMaxStages; ++i) {
+    p.radix[i] = i < p.nstages ? a.radix[i] : 0;
+    p.f_ns[i] = make_fastdiv(ns);
+    p.f_nr[i] = make_fastdiv(i < p.nstages ? a.n / a.radix[i] : 1);
+    if (i < p.nstages) ns *= a.radix[i];
+  }
+  p.f_n = make_fastdiv(a.n);
   p.tw = a.d_tw;
   p.tw64 = a.d_tw64;
   return p;
@@ -104,9 +111,17 @@ constexpr size_t kMaxLds = 60 * 1024;
 
 struct Tile {
   int lines = 0;  // Lz (z), T (y) or Tm (x)
-  int E = 0;
+  int E = 0;      // register elements per thread
+  int NT = 0;     // threads per block
   size_t lds = 0;
 };
+
+struct Budget {
+  int E, NT;
+};
+// preference order: fewest register elements per thread first
+const std::vector<Budget> kBudget32 = {{8, 256}, {8, 512}, {16, 512}, {16, 256}, {32, 256}, {64, 256}};
+const std::vector<Budget> kBudget64 = {{8, 256}, {8, 512}, {16, 256}, {32, 256}};
 
 int env_int(const char* name, int dflt) {
   const char* v = std::getenv(name);
@@ -128,6 +143,7 @@ struct ls_solver {
   Tile tz64, ty64, tx64;  // FP64 base evaluation passes
   int nblk_z = 0, nblk_y = 0, ntiles_y = 0, nblk_x = 0;
   int nblk_z64 = 0, nblk_y64 = 0, ntiles_y64 = 0, nblk_x64 = 0;
+  DbLayout dl;
   float2* spec = nullptr;
   double2* spec64 = nullptr;  // one load case
   float* base = nullptr;
@@ -199,16 +215,17 @@ int build_axis(HostAxis& a, int n) {
 }
 
 // Largest tile of at most `want` units whose FFT fits one of the register
-// budgets `Es` with LDS <= kMaxLds; a unit is `lines_per_unit` FFT lines of
+// budgets with LDS <= kMaxLds; a unit is `lines_per_unit` FFT lines of
 // `bytes_per_unit` LDS bytes.
-bool choose_tile(const HostAxis& a, int want, int lines_per_unit, size_t bytes_per_unit, const std::vector<int>& Es,
-                 Tile& t) {
+bool choose_tile(const HostAxis& a, int want, int lines_per_unit, size_t bytes_per_unit,
+                 const std::vector<Budget>& budgets, Tile& t) {
   for (int u = std::max(1, want); u >= 1; --u) {
     if ((size_t)u * bytes_per_unit > kMaxLds) continue;
-    for (int E : Es) {
-      if (fits(a, u * lines_per_unit, E)) {
+    for (const Budget& b : budgets) {
+      if (fits(a, u * lines_per_unit, b.E, b.NT)) {
         t.lines = u;
-        t.E = E;
+        t.E = b.E;
+        t.NT = b.NT;
         t.lds = (size_t)u * bytes_per_unit;
         return true;
       }
@@ -216,6 +233,24 @@ bool choose_tile(const HostAxis& a, int want, int lines_per_unit, size_t bytes_p
   }
   return false;
 }
+
+// Expand CALL(E, NT) for the (E, NT) pair of a tile.
+#define LS_DISPATCH32(TILE, CALL)          \
+  switch ((TILE).E * 1000 + (TILE).NT) {   \
+    case 8256: CALL(8, 256); break;        \
+    case 8512: CALL(8, 512); break;        \
+    case 16512: CALL(16, 512); break;      \
+    case 16256: CALL(16, 256); break;      \
+    case 32256: CALL(32, 256); break;      \
+    default: CALL(64, 256); break;         \
+  }
+#define LS_DISPATCH64(TILE, CALL)          \
+  switch ((TILE).E * 1000 + (TILE).NT) {   \
+    case 8256: CALL(8, 256); break;        \
+    case 8512: CALL(8, 512); break;        \
+    case 16256: CALL(16, 256); break;      \
+    default: CALL(32, 256); break;         \
+  }
 
 void mark(ls_solver* s, int kind) {
   if (!s->timing) return;
@@ -237,61 +272,40 @@ void zline(ls_solver* s, const float* lam, const float* mu, float* eps) {
   aa.head = s->aa_head;
   aa.filled = s->aa_filled;
   aa.m1 = s->m1;
-  dim3 grid(s->nblk_z, s->nrhs);
+  const unsigned grid = (unsigned)s->nblk_z * s->nrhs;
   const size_t lds = s->tz.lds;
-  switch (s->tz.E) {
-#define LS_ZCASE(EV)                                                                                       \
-  case EV:                                                                                                 \
-    k_zline<MODE, EV><<<grid, kThreads, lds, s->stream>>>(s->gd, s->d_rhs, s->d_st, lam, mu, eps, s->spec, \
-                                                          s->tz.lines, aa);                                \
-    break;
-    LS_ZCASE(8)
-    LS_ZCASE(16)
-    LS_ZCASE(32)
-    LS_ZCASE(64)
-#undef LS_ZCASE
-  }
+#define LS_Z(EV, NTV)                                                                                           \
+  k_zline<MODE, EV, NTV><<<grid, NTV, lds, s->stream>>>(s->gd, s->d_rhs, s->d_st, lam, mu, eps, s->spec, s->tz.lines, \
+                                                        s->nblk_z, aa)
+  LS_DISPATCH32(s->tz, LS_Z)
+#undef LS_Z
   mark(s, 0);
 }
 
 void ypass(ls_solver* s, bool inv) {
-  dim3 grid(s->nblk_y, s->nrhs);
+  const unsigned grid = (unsigned)s->nblk_y * s->nrhs;
   const size_t lds = s->ty.lds;
-  switch (s->ty.E) {
-#define LS_YCASE(EV)                                                                                      \
-  case EV:                                                                                                \
-    if (inv)                                                                                              \
-      k_ypass<true, EV, float2><<<grid, kThreads, lds, s->stream>>>(s->gd, s->d_st, s->spec, s->ty.lines, \
-                                                                     s->ntiles_y, -1);                   \
-    else                                                                                                  \
-      k_ypass<false, EV, float2><<<grid, kThreads, lds, s->stream>>>(s->gd, s->d_st, s->spec, s->ty.lines, \
-                                                                      s->ntiles_y, -1);                  \
-    break;
-    LS_YCASE(8)
-    LS_YCASE(16)
-    LS_YCASE(32)
-    LS_YCASE(64)
-#undef LS_YCASE
-  }
+  const FastDiv fT = make_fastdiv(s->ty.lines);
+#define LS_Y(EV, NTV)                                                                                               \
+  if (inv)                                                                                                          \
+    k_ypass<true, EV, NTV, float2><<<grid, NTV, lds, s->stream>>>(s->gd, s->d_st, s->spec, fT, s->ntiles_y, -1);   \
+  else                                                                                                              \
+    k_ypass<false, EV, NTV, float2><<<grid, NTV, lds, s->stream>>>(s->gd, s->d_st, s->spec, fT, s->ntiles_y, -1)
+  LS_DISPATCH32(s->ty, LS_Y)
+#undef LS_Y
   mark(s, 1);
 }
 
 void xpass(ls_solver* s, int do_gamma, bool with_base = true) {
-  dim3 grid(s->nblk_x, s->nrhs);
+  const unsigned grid = (unsigned)s->nblk_x * s->nrhs;
   const size_t lds = s->tx.lds;
   const float2* db = with_base ? s->dbase : nullptr;
-  switch (s->tx.E) {
-#define LS_XCASE(EV)                                                                                           \
-  case EV:                                                                                                     \
-    k_xpass<EV><<<grid, kThreads, lds, s->stream>>>(s->gd, s->d_rhs, s->d_st, s->spec, db, s->partials, s->dc, \
-                                                     s->tx.lines, do_gamma);                                   \
-    break;
-    LS_XCASE(8)
-    LS_XCASE(16)
-    LS_XCASE(32)
-    LS_XCASE(64)
-#undef LS_XCASE
-  }
+  const FastDiv fTm = make_fastdiv(s->tx.lines), fcols = make_fastdiv(6 * s->tx.lines);
+#define LS_X(EV, NTV)                                                                                      \
+  k_xpass<EV, NTV><<<grid, NTV, lds, s->stream>>>(s->gd, s->d_rhs, s->d_st, s->spec, db, s->partials, s->dc, \
+                                                  fTm, fcols, s->nblk_x, s->dl, do_gamma)
+  LS_DISPATCH32(s->tx, LS_X)
+#undef LS_X
   mark(s, 2);
 }
 
@@ -302,40 +316,23 @@ void check(ls_solver* s, double rtol, double atol, int mode) {
 
 // FP64 evaluation of d_base for load case r from s->base (+tau).
 void base_eval_one(ls_solver* s, int r, const float* lam, const float* mu, const float* tau) {
-  switch (s->tz64.E) {
-#define LS_BZ(EV)                                                                                              \
-  case EV:                                                                                                     \
-    k_base_z<EV><<<s->nblk_z64, kThreads, s->tz64.lds, s->stream>>>(s->gd, s->d_rhs, r, lam, mu, s->base, tau, \
-                                                                    s->spec64, s->tz64.lines);                 \
-    break;
-    LS_BZ(8)
-    LS_BZ(16)
-    LS_BZ(32)
+#define LS_BZ(EV, NTV)                                                                                          \
+  k_base_z<EV, NTV><<<s->nblk_z64, NTV, s->tz64.lds, s->stream>>>(s->gd, s->d_rhs, r, lam, mu, s->base, tau, \
+                                                                  s->spec64, s->tz64.lines)
+  LS_DISPATCH64(s->tz64, LS_BZ)
 #undef LS_BZ
-  }
-  switch (s->ty64.E) {
-#define LS_BY(EV)                                                                                              \
-  case EV:                                                                                                     \
-    k_ypass<false, EV, double2><<<s->nblk_y64, kThreads, s->ty64.lds, s->stream>>>(s->gd, s->d_st, s->spec64, \
-                                                                                    s->ty64.lines,           \
-                                                                                    s->ntiles_y64, r);       \
-    break;
-    LS_BY(8)
-    LS_BY(16)
-    LS_BY(32)
+  const FastDiv fT = make_fastdiv(s->ty64.lines);
+#define LS_BY(EV, NTV)                                                                                        \
+  k_ypass<false, EV, NTV, double2><<<s->nblk_y64, NTV, s->ty64.lds, s->stream>>>(s->gd, s->d_st, s->spec64,   \
+                                                                                fT, s->ntiles_y64, r)
+  LS_DISPATCH64(s->ty64, LS_BY)
 #undef LS_BY
-  }
-  switch (s->tx64.E) {
-#define LS_BX(EV)                                                                                           \
-  case EV:                                                                                                  \
-    k_base_x<EV><<<s->nblk_x64, kThreads, s->tx64.lds, s->stream>>>(s->gd, r, s->spec64, s->dbase, s->d_st, \
-                                                                    s->tx64.lines);                         \
-    break;
-    LS_BX(8)
-    LS_BX(16)
-    LS_BX(32)
+  const FastDiv fTm = make_fastdiv(s->tx64.lines), fcols = make_fastdiv(6 * s->tx64.lines);
+#define LS_BX(EV, NTV)                                                                                         \
+  k_base_x<EV, NTV><<<s->nblk_x64, NTV, s->tx64.lds, s->stream>>>(s->gd, r, s->spec64, s->dbase, s->d_st,     \
+                                                                  fTm, fcols, s->dl)
+  LS_DISPATCH64(s->tx64, LS_BX)
 #undef LS_BX
-  }
   mark(s, 3);
 }
 
@@ -440,18 +437,18 @@ int create_impl(ls_solver* s, const int* voxels, const float* extents, int nrhs,
   for (int d = 0; d < 3; ++d)
     if (build_axis(s->ax[d], n[d])) return -1;
   // tiles: aim for ~3-6k complex elements per block (overridable for tuning)
-  const std::vector<int> E32 = {8, 16, 32, 64}, E64 = {8, 16, 32};
   const int wz = env_int("LS_TILE_Z", std::max(1, 3072 / (3 * s->nz)));
   const int wy = env_int("LS_TILE_Y", std::max(1, std::min(6 * s->P, 4096 / s->ny)));
-  const int wx = env_int("LS_TILE_X", std::max(1, 6144 / (6 * s->nx)));
-  const bool ok32 = choose_tile(s->ax[2], wz, 3, (size_t)3 * s->nz * sizeof(float2), E32, s->tz) &&
-                    choose_tile(s->ax[1], wy, 1, (size_t)s->ny * sizeof(float2), E32, s->ty) &&
-                    choose_tile(s->ax[0], wx, 6, (size_t)6 * s->nx * sizeof(float2), E32, s->tx);
+  const int wx = env_int("LS_TILE_X", std::max(1, 3072 / (6 * s->nx)));
+  const int lsz = s->nz + (s->nz >> 4) + 1;  // padded z-line LDS stride
+  const bool ok32 = choose_tile(s->ax[2], wz, 3, (size_t)3 * lsz * sizeof(float2), kBudget32, s->tz) &&
+                    choose_tile(s->ax[1], wy, 1, (size_t)s->ny * sizeof(float2), kBudget32, s->ty) &&
+                    choose_tile(s->ax[0], wx, 6, (size_t)6 * s->nx * sizeof(float2), kBudget32, s->tx);
   const bool ok64 =
-      choose_tile(s->ax[2], std::max(1, 2048 / (3 * s->nz)), 3, (size_t)3 * s->nz * sizeof(double2), E64, s->tz64) &&
-      choose_tile(s->ax[1], std::max(1, std::min(6 * s->P, 2048 / s->ny)), 1, (size_t)s->ny * sizeof(double2), E64,
+      choose_tile(s->ax[2], std::max(1, 2048 / (3 * s->nz)), 3, (size_t)3 * lsz * sizeof(double2), kBudget64, s->tz64) &&
+      choose_tile(s->ax[1], std::max(1, std::min(6 * s->P, 2048 / s->ny)), 1, (size_t)s->ny * sizeof(double2), kBudget64,
                   s->ty64) &&
-      choose_tile(s->ax[0], std::max(1, 2048 / (6 * s->nx)), 6, (size_t)6 * s->nx * sizeof(double2), E64, s->tx64);
+      choose_tile(s->ax[0], std::max(1, 2048 / (6 * s->nx)), 6, (size_t)6 * s->nx * sizeof(double2), kBudget64, s->tx64);
   if (!ok32 || !ok64) {
     fprintf(stderr, "liblippmannschwinger: no FFT tile fits for grid %d x %d x %d\n", s->nx, s->ny, s->nz);
     return -1;
@@ -483,8 +480,14 @@ int create_impl(ls_solver* s, const int* voxels, const float* extents, int nrhs,
   g.px = to_dev(s->ax[0]);
   g.py = to_dev(s->ax[1]);
   g.pz = to_dev(s->ax[2]);
+  g.f_nz = make_fastdiv(s->nz);
+  g.f_P = make_fastdiv(s->P);
+  g.lsz = lsz;
+  s->dl.f_Tm = make_fastdiv(s->tx.lines);
+  s->dl.nx = s->nx;
+  s->dl.per_rhs = (long long)s->nblk_x * s->nx * s->tx.lines * 3;
   if (dalloc(&s->spec, (size_t)nrhs * s->M * 6) || dalloc(&s->spec64, (size_t)s->M * 6) ||
-      dalloc(&s->base, (size_t)nrhs * 6 * s->N) || dalloc(&s->dbase, (size_t)nrhs * s->M * 3) ||
+      dalloc(&s->base, (size_t)nrhs * 6 * s->N) || dalloc(&s->dbase, (size_t)nrhs * s->dl.per_rhs) ||
       dalloc(&s->partials, (size_t)nrhs * s->nblk_x) || dalloc(&s->dc, (size_t)nrhs * 6) ||
       dalloc(&s->d_rhs, nrhs) || dalloc(&s->d_st, nrhs) || dalloc(&s->d_sel, nrhs))
     return -1;
