@@ -23,33 +23,48 @@ sys.path.insert(0, ROOT)
 METRIC = "C_eff solves/sec to 1e-6 residual at 256³ (plain vs batched AA + learned start)"
 
 
-def model_bytes(kind, N, M, iters, depth, nrhs_share=6):
-    """Algorithmic HBM bytes of all launches of one pass kind over a solve.
+def model_bytes(kind, N, M, iters, depth, nrebase=0):
+    """Algorithmic HBM bytes of all launches of one kernel kind over a solve.
 
-    Per load-case iteration (one HBM round trip of the spectrum per pass):
-      z pass plain: spectrum 96 M + strain 48 N + Lame 8 N
-      z passes Anderson(m) (history full): 96 M + (128 + 48 m) N (minus the 56 N above)
-      y passes (forward + inverse): 2 x 96 M;  x pass: 96 M.
-    The Lame fields are shared by the batched load cases: counted once per
-    iteration of the batch (8 N per launch).
+    One HBM round trip of the z-half spectrum (48 M bytes per load case) per
+    pass and direction; per load case and iteration j (f_j = min(j, depth+1)
+    filled Anderson slots):
+      z plain           : spectrum 96 M + strain 48 N (+ Lame 8 N shared by the batch)
+      z Anderson resid. : spectrum 48 M + strain 24 N + r, g written 48 N + r history (f_j - 1) 24 N
+      z Anderson mix    : g history f_j 24 N + strain 24 N + spectrum 48 M (+ Lame 8 N shared)
+      y (inverse + forward) : 2 x 96 M;  x : 96 M (setup, every check, every re-evaluation)
     """
-    its = list(iters)
-    upd = sum(its)
-    chk = sum(i + 1 for i in its)
+    its = [int(i) for i in iters]
+    m1 = depth + 1
     nb = max(its) if its else 0
     if kind == "z":
-        init = len(its) * (24 * N + 48 * M) + 8 * N
-        if depth == 0:
-            per = upd * (96 * M + 48 * N) + nb * 8 * N
-        else:
-            m = depth
-            per = upd * (96 * M + (120 + 48 * m) * N) + nb * 8 * N
-        return init + per
+        return sum(k * (96 * M + 48 * N) for k in its) + nb * 8 * N if depth == 0 else 0
+    if kind == "z_aa_residual":
+        return sum(48 * M + 48 * N + 24 * N + 24 * N * (min(j, m1) - 1) for k in its for j in range(1, k + 1))
+    if kind == "z_aa_mix":
+        return sum(24 * N * min(j, m1) + 24 * N + 48 * M for k in its for j in range(1, k + 1)) + nb * 8 * N
     if kind == "y":
-        return (chk + upd) * 96 * M
+        return sum(2 * k for k in its) * 96 * M
     if kind == "x":
-        return chk * 96 * M
+        return (sum(k + 1 for k in its) + nrebase * len(its)) * 96 * M
     return 0
+
+
+KERNEL_OF = {"z": "k_zline<Z_UPD>", "z_aa_residual": "k_zline<Z_AAINV>", "z_aa_mix": "k_zline<Z_AAMIX>",
+             "y": "k_ypass", "x": "k_xpass"}
+
+
+def measured_traffic(kind, n, method):
+    """HBM bytes per launch (read + write) of a kernel kind from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (profiles/traffic.txt), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 5 and not line.startswith("#") and f[0] == kind and f[1] == str(n) and f[2] == method:
+            return float(f[3]) + float(f[4])
+    return None
 
 
 def cpu_baseline_sample(lam, mu, gs, gpu_iters, timed=2):
@@ -144,37 +159,45 @@ def main():
         for _ in range(args.warmup):
             solver.solve(lam, mu, loads, rtol=args.rtol, maxiter=3000)
         torch.cuda.synchronize()
-        times, ms_kind, cnt_kind = [], np.zeros(4), np.zeros(4, dtype=np.int64)
+        times = []
+        ms_kind = {k: 0.0 for k in LSSolver.KINDS}
+        cnt_kind = {k: 0 for k in LSSolver.KINDS}
         out = None
+        nreb = 0
         for _ in range(args.steps):
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             out = solver.solve(lam, mu, loads, rtol=args.rtol, maxiter=3000)
             torch.cuda.synchronize()
             times.append(time.perf_counter() - t0)
-            ms, cnt = solver.last_timing()
-            ms_kind += ms
-            cnt_kind += cnt
+            ms, cnt, nreb = solver.last_timing()
+            for k in LSSolver.KINDS:
+                ms_kind[k] += ms[k]
+                cnt_kind[k] += cnt[k]
         iters = [int(i) for i in out["iters"]]
         C = out["sigma_mean"]
         sec = float(np.mean(times))
         kinds = {}
-        for k, name in enumerate(["z", "y", "x"]):
-            b = model_bytes(name, N, M, iters, depth) * args.steps
-            t = ms_kind[k] / 1e3
-            kinds[name] = {"ms_total": ms_kind[k] / args.steps, "launches": int(cnt_kind[k] // args.steps),
-                           "algorithmic_bytes": b / args.steps,
-                           "achieved_GBps": (b / t / 1e9) if t > 0 else None}
-        loop_bytes = sum(model_bytes(k, N, M, iters, depth) for k in ("z", "y", "x"))
+        for name in ("z", "z_aa_residual", "z_aa_mix", "y", "x"):
+            if cnt_kind[name] == 0:
+                continue
+            b = model_bytes(name, N, M, iters, depth, nreb)
+            t = ms_kind[name] / 1e3 / args.steps
+            kinds[name] = {"kernel": KERNEL_OF[name], "ms_total": t * 1e3,
+                           "launches": int(cnt_kind[name] // args.steps),
+                           "avg_launch_ms": t * 1e3 / max(1, cnt_kind[name] // args.steps),
+                           "algorithmic_bytes": b, "achieved_GBps": (b / t / 1e9) if t > 0 else None}
+        loop_bytes = sum(model_bytes(k, N, M, iters, depth, nreb) for k in kinds)
         rec = {"solves_per_s": 1.0 / sec, "ms_per_solve": sec * 1e3, "iters": iters, "depth": depth,
-               "passes": kinds, "loop_GBps": loop_bytes / sec / 1e9, "ceff": C.tolist()}
+               "base_reevaluations": nreb, "passes": kinds, "loop_GBps": loop_bytes / sec / 1e9,
+               "ceff": C.tolist()}
         if golden is not None:
             sig = np.abs(golden) > 1e-3 * np.abs(golden).max()
             rec["ceff_max_rel_err_vs_fp64_oracle"] = float((np.abs(C - golden)[sig] / np.abs(golden)[sig]).max())
         results[method] = rec
         print(f"# {method}: {rec['ms_per_solve']:.1f} ms/solve, iters {iters}, loop {rec['loop_GBps']:.0f} GB/s, "
-              + ", ".join(f"{k} {v['achieved_GBps']:.0f} GB/s" for k, v in kinds.items() if v['achieved_GBps']),
-              file=sys.stderr, flush=True)
+              + ", ".join(f"{k} {v['achieved_GBps']:.0f} GB/s ({v['ms_total']:.0f} ms)" for k, v in kinds.items()
+                          if v['achieved_GBps']), file=sys.stderr, flush=True)
         solver.close()
         del solver
         torch.cuda.empty_cache()
@@ -184,14 +207,11 @@ def main():
     dom = max(best["passes"], key=lambda k: best["passes"][k]["ms_total"])
     dk = best["passes"][dom]
     peak = 8000.0
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_{dom}pass_{n}.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = json.load(f).get("bytes_per_launch")
-    roofline = {"bound": "hbm", "kernel": f"k_{dom}{'line' if dom == 'z' else 'pass'}",
-                "achieved": dk["achieved_GBps"], "peak": peak, "unit": "GB/s",
-                "frac": dk["achieved_GBps"] / peak, "traffic": traffic,
+    roofline = {"bound": "hbm", "kernel": dk["kernel"], "achieved": dk["achieved_GBps"], "peak": peak,
+                "unit": "GB/s", "frac": dk["achieved_GBps"] / peak,
+                "traffic": measured_traffic(dom, n, best_name),
+                "algorithmic_bytes_per_launch": dk["algorithmic_bytes"] / max(1, dk["launches"]),
+                "avg_launch_ms": dk["avg_launch_ms"],
                 "loop_achieved": best["loop_GBps"], "loop_frac": best["loop_GBps"] / peak}
     cpu = None
     if not args.no_cpu_baseline and "plain" in results:

@@ -74,9 +74,13 @@ int ls_solver_solve(ls_solver *s, const float *d_lambda, const float *d_mu,
                     double rel_ref, int verbose, int *iters, double *residual,
                     double *sigma_mean);
 
-/* Kernel time of the last solve, ms per pass kind: [0] z-line passes,
- * [1] y passes, [2] x passes, [3] everything else; and pass counts. */
-int ls_solver_last_timing(ls_solver *s, double *ms4, int *count4);
+/* Kernel time of the last solve (recorded with HIP events on the solver's
+ * stream when the environment variable LS_TIMING is set), ms and launch
+ * counts per pass kind: [0] z-line passes (plain), [1] y passes, [2] x passes,
+ * [3] everything else, [4] z-line Anderson residual passes, [5] z-line
+ * Anderson mixing passes.  Fills the first nkinds kinds; returns the number
+ * of base re-evaluations of the last solve. */
+int ls_solver_last_timing(ls_solver *s, double *ms, int *count, int nkinds);
 
 void ls_solver_destroy(ls_solver *s);
 

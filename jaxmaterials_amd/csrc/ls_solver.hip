@@ -172,8 +172,9 @@ struct ls_solver {
   int* aa_head = nullptr;
   int* aa_filled = nullptr;
   // timing of the last solve (LS_TIMING set): HIP events on the solver stream
-  double t_ms[4] = {0, 0, 0, 0};
-  int t_cnt[4] = {0, 0, 0, 0};
+  static constexpr int kKinds = 6;  // z (plain), y, x, other, z Anderson residual, z Anderson mix
+  double t_ms[kKinds] = {0, 0, 0, 0, 0, 0};
+  int t_cnt[kKinds] = {0, 0, 0, 0, 0, 0};
   bool timing = false;
   std::vector<hipEvent_t> tev;
   std::vector<int> tkind;
@@ -279,7 +280,7 @@ void zline(ls_solver* s, const float* lam, const float* mu, float* eps) {
                                                         s->nblk_z, aa)
   LS_DISPATCH32(s->tz, LS_Z)
 #undef LS_Z
-  mark(s, 0);
+  mark(s, MODE == Z_AAINV ? 4 : (MODE == Z_AAMIX ? 5 : 0));
 }
 
 void ypass(ls_solver* s, bool inv) {
@@ -775,7 +776,7 @@ int solve_impl(ls_solver* s, const float* d_lambda, const float* d_mu, const int
   LS_HIP(hipStreamSynchronize(s->stream));
 
   if (s->timing) {
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < ls_solver::kKinds; ++k) {
       s->t_ms[k] = 0.0;
       s->t_cnt[k] = 0;
     }
@@ -835,11 +836,11 @@ int ls_solver_solve(ls_solver* s, const float* d_lambda, const float* d_mu, cons
                     rel_ref, verbose, iters, residual, sigma_mean, false);
 }
 
-int ls_solver_last_timing(ls_solver* s, double* ms4, int* count4) {
+int ls_solver_last_timing(ls_solver* s, double* ms, int* count, int nkinds) {
   if (!s) return -1;
-  for (int k = 0; k < 4; ++k) {
-    if (ms4) ms4[k] = s->t_ms[k];
-    if (count4) count4[k] = s->t_cnt[k];
+  for (int k = 0; k < nkinds && k < ls_solver::kKinds; ++k) {
+    if (ms) ms[k] = s->t_ms[k];
+    if (count) count[k] = s->t_cnt[k];
   }
   return s->n_rebase;
 }

@@ -114,11 +114,14 @@ class LSSolver:
         return {"epsilon": epsilon, "sigma": sigma, "iters": iters, "residual": res, "sigma_mean": smean,
                 "converged": rc == 0}
 
+    KINDS = ("z", "y", "x", "other", "z_aa_residual", "z_aa_mix")
+
     def last_timing(self):
-        """Kernel milliseconds of the last solve by pass kind (z, y, x, other) when
-        the environment variable LS_TIMING was set, and the pass counts."""
-        ms = np.zeros(4, dtype=np.float64)
-        cnt = np.zeros(4, dtype=np.int32)
-        self.lib.ls_solver_last_timing(self._h, ms.ctypes.data_as(ctypes.c_void_p),
-                                       cnt.ctypes.data_as(ctypes.c_void_p))
-        return ms, cnt
+        """Kernel milliseconds and launch counts of the last solve by pass kind
+        (``LSSolver.KINDS``), recorded when the environment variable LS_TIMING
+        was set; also the number of base re-evaluations."""
+        ms = np.zeros(6, dtype=np.float64)
+        cnt = np.zeros(6, dtype=np.int32)
+        nreb = self.lib.ls_solver_last_timing(self._h, ms.ctypes.data_as(ctypes.c_void_p),
+                                              cnt.ctypes.data_as(ctypes.c_void_p), 6)
+        return dict(zip(self.KINDS, ms.tolist())), dict(zip(self.KINDS, cnt.tolist())), int(nreb)
