@@ -64,15 +64,18 @@ ls_solver *ls_solver_create(const int *voxels, const float *extents, int nrhs,
  * case), eps_bar[nrhs][6]; outputs iters[nrhs], residual[nrhs] (final
  * relative residual), sigma_mean[nrhs][6] (FP64 volume averages), each may
  * be NULL.  The relative tolerance is measured against the residual of the
- * uniform start R(eps_bar) in every case; rel_ref > 0 overrides it (used by
- * the adjoint solve).  Returns 0 if all converged, 1 if some did not, -1 on
- * error. */
+ * uniform start R(eps_bar) in every case; rel_ref > 0 overrides it.
+ * flags: LS_ABSOLUTE_RESIDUAL = residual ||xi . sigma_hat|| without the
+ * division by ||sigma_hat(0)|| (for polarisation-driven solves such as the
+ * adjoint, whose stress may have zero mean).  Returns 0 if all converged,
+ * 1 if some did not, -1 on error. */
+#define LS_ABSOLUTE_RESIDUAL 1
 int ls_solver_solve(ls_solver *s, const float *d_lambda, const float *d_mu,
                     const int *rhs_mat, const float *eps_bar,
                     const float *d_tau, float *d_eps, float *d_sigma,
                     int warm_start, float rtol, float atol, int maxiter,
-                    double rel_ref, int verbose, int *iters, double *residual,
-                    double *sigma_mean);
+                    double rel_ref, int flags, int verbose, int *iters,
+                    double *residual, double *sigma_mean);
 
 /* Kernel time of the last solve (recorded with HIP events on the solver's
  * stream when the environment variable LS_TIMING is set), ms and launch

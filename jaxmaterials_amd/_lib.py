@@ -29,7 +29,7 @@ SIGNATURES = {
     "ls_solver_solve": (
         ctypes.c_int,
         [ctypes.c_void_p] * 8 + [ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_double,
-                                 ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+                                 ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     ),
     "ls_solver_last_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
     "ls_solver_destroy": (None, [ctypes.c_void_p]),

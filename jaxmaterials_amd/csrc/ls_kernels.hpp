@@ -611,9 +611,11 @@ __global__ void __launch_bounds__(NT) k_base_x(GridDev g, int r, const double2* 
 // R < max(rtol R0, atol), R0 = residual of the first check unless preset).
 // mode 0: a new check; 1: record R only (no stop test); 2: re-evaluation of
 // the current check after a rebase (not counted as a new check).
+// absolute != 0: R = ||d|| without the ||sigma_hat(0)|| normalisation (the
+// adjoint solve, whose stress can have zero mean).
 __global__ void __launch_bounds__(kThreads) k_check(RhsState* __restrict__ st, const double* __restrict__ partials,
                                                     const float* __restrict__ dc, int nparts, double rtol,
-                                                    double atol, int mode) {
+                                                    double atol, int mode, int absolute) {
   __shared__ double red[kThreads / 64];
   const int r = blockIdx.x;
   if (!st[r].active) return;
@@ -628,7 +630,7 @@ __global__ void __launch_bounds__(kThreads) k_check(RhsState* __restrict__ st, c
     for (int c = 0; c < 6; ++c) z6[c] = q.dcb[c] + (double)z[c];
     const double den = sqrt(z6[0] * z6[0] + z6[1] * z6[1] + z6[2] * z6[2] +
                             2.0 * (z6[3] * z6[3] + z6[4] * z6[4] + z6[5] * z6[5]));
-    const double R = sqrt(s) / den;
+    const double R = absolute ? sqrt(s) : sqrt(s) / den;
     q.R = R;
     q.den = den;
     if (!(R == R) || R > 1e300) {

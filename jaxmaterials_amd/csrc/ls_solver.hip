@@ -163,6 +163,7 @@ struct ls_solver {
   int nblk_mm = 0, chunk_mm = 0;
   double rebase_factor = 1e-3;
   int n_rebase = 0;
+  int absolute = 0;  // LS_ABSOLUTE_RESIDUAL flag of the current solve
   // Anderson
   float* aaG = nullptr;
   float* aaR = nullptr;
@@ -311,7 +312,8 @@ void xpass(ls_solver* s, int do_gamma, bool with_base = true) {
 }
 
 void check(ls_solver* s, double rtol, double atol, int mode) {
-  k_check<<<s->nrhs, kThreads, 0, s->stream>>>(s->d_st, s->partials, s->dc, s->nblk_x, rtol, atol, mode);
+  k_check<<<s->nrhs, kThreads, 0, s->stream>>>(s->d_st, s->partials, s->dc, s->nblk_x, rtol, atol, mode,
+                                               s->absolute);
   mark(s, 3);
 }
 
@@ -571,7 +573,8 @@ int rebase(ls_solver* s, const float* lam, const float* mu, const float* tau, fl
 int solve_impl(ls_solver* s, const float* d_lambda, const float* d_mu, const int* rhs_mat, const float* eps_bar,
                const float* d_tau, float* d_eps, float* d_sigma, int warm_start, float rtol, float atol,
                int maxiter, double rel_ref, int verbose, int* iters, double* residual, double* sigma_mean,
-               bool legacy_q7) {
+               bool legacy_q7, int flags = 0) {
+  s->absolute = (flags & 1) ? 1 : 0;
   if (maxiter < 0) {
     fprintf(stderr, "liblippmannschwinger: maxiter must be >= 0\n");
     return -1;
@@ -829,11 +832,11 @@ void ls_solver_destroy(ls_solver* s) { free_solver(s); }
 
 int ls_solver_solve(ls_solver* s, const float* d_lambda, const float* d_mu, const int* rhs_mat,
                     const float* eps_bar, const float* d_tau, float* d_eps, float* d_sigma, int warm_start,
-                    float rtol, float atol, int maxiter, double rel_ref, int verbose, int* iters, double* residual,
-                    double* sigma_mean) {
+                    float rtol, float atol, int maxiter, double rel_ref, int flags, int verbose, int* iters,
+                    double* residual, double* sigma_mean) {
   if (!s || !d_lambda || !d_mu || !eps_bar || !d_eps) return -1;
   return solve_impl(s, d_lambda, d_mu, rhs_mat, eps_bar, d_tau, d_eps, d_sigma, warm_start, rtol, atol, maxiter,
-                    rel_ref, verbose, iters, residual, sigma_mean, false);
+                    rel_ref, verbose, iters, residual, sigma_mean, false, flags);
 }
 
 int ls_solver_last_timing(ls_solver* s, double* ms, int* count, int nkinds) {

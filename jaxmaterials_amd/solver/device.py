@@ -70,7 +70,8 @@ class LSSolver:
             pass
 
     def solve(self, lmbda, mu, epsilon_bar, rhs_mat=None, tau=None, epsilon=None, warm_start=False,
-              want_sigma=False, rtol=1e-6, atol=0.0, maxiter=1000, rel_ref=0.0, verbose=0, check=True):
+              want_sigma=False, rtol=1e-6, atol=0.0, maxiter=1000, rel_ref=0.0, absolute_residual=False,
+              verbose=0, check=True):
         """Solve all load cases.
 
         :arg lmbda, mu: GPU float32 [nmat, nx, ny, nz] (or [nx, ny, nz] when nmat == 1)
@@ -79,6 +80,7 @@ class LSSolver:
         :arg tau: optional GPU float32 [nrhs, 6, nx, ny, nz] polarisation added to the stress
         :arg epsilon: optional GPU float32 [nrhs, 6, nx, ny, nz]; start field if warm_start
         :arg rel_ref: normaliser of rtol; 0 means the residual of the uniform start
+        :arg absolute_residual: residual without the mean-stress normalisation
         Returns a dict with epsilon, sigma (or None), iters, residual, sigma_mean (float64 [nrhs, 6]).
         """
         R = self.nrhs
@@ -104,7 +106,7 @@ class LSSolver:
         rc = self.lib.ls_solver_solve(
             self._h, _ptr(lm), _ptr(m), mats.ctypes.data_as(ctypes.c_void_p), eb.ctypes.data_as(ctypes.c_void_p),
             _ptr(tau), _ptr(epsilon), _ptr(sigma), int(bool(warm_start)), float(rtol), float(atol), int(maxiter),
-            float(rel_ref), int(verbose), iters.ctypes.data_as(ctypes.c_void_p),
+            float(rel_ref), 1 if absolute_residual else 0, int(verbose), iters.ctypes.data_as(ctypes.c_void_p),
             res.ctypes.data_as(ctypes.c_void_p), smean.ctypes.data_as(ctypes.c_void_p),
         )
         if rc < 0:

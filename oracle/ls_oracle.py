@@ -494,3 +494,31 @@ def material_properties(grid_spec, fibre_positions, fibre_radii, fibre_orientati
                 for sh in shifts:
                     data[j] += ((coord - p[0] - sh) ** 2 + (Z - p[1]) ** 2 < r**2) * (values_fibre[j] - vm)
     return data
+
+
+def ceff_gradient_closed_form(eps_fields, dL_dC):
+    """Gradient of a loss on C_eff with respect to the Lame fields, from the six
+    converged strain fields alone (the build's closed form; FD-pinned in tests).
+
+    By Hill-Mandel, W(a, b) = <eps^a : C : eps^b> = w_a C_eff[b][a] (tensor-shear
+    Voigt weights w), and at equilibrium the variation of W with C(x) is
+    eps^a(x) : dC(x) : eps^b(x) / N.  Hence
+        dC_eff[b][a]/dlambda(x) = tr eps^a tr eps^b / (N w_a),
+        dC_eff[b][a]/dmu(x)     = 2 <eps^a, eps^b>_w / (N w_a).
+    eps_fields: [6, 6, nx, ny, nz] (load case, component); dL_dC: [6, 6].
+    Returns (dL/dlambda, dL/dmu) as float64 arrays [nx, ny, nz].
+    """
+    e = np.asarray(eps_fields, dtype=np.float64)
+    N = e[0, 0].size
+    tr = e[:, 0] + e[:, 1] + e[:, 2]
+    dl = np.zeros(e.shape[2:])
+    dm = np.zeros(e.shape[2:])
+    W = VOIGT_WEIGHTS
+    for b in range(6):
+        for a in range(6):
+            c = dL_dC[b, a] / (N * W[a])
+            if c == 0.0:
+                continue
+            dl += c * tr[a] * tr[b]
+            dm += c * 2.0 * np.einsum("c,c...->...", W, e[a] * e[b])
+    return dl, dm
