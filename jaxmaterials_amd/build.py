@@ -15,9 +15,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB_NAME = "liblippmannschwinger.so"
 LIB_PATH = os.path.join(HERE, LIB_NAME)
 
-SOURCES = ["ls_solver.hip", "microstructure.hip"]
-CONV_SOURCES = ["conv3d.hip"]
-CONV_LIB = os.path.join(HERE, "libls_warmstart.so")
+# every HIP source of the library; each compiles to its own object in parallel
+SOURCES = ["ls_solver.hip", "microstructure.hip", "conv3d.hip"]
 
 HIPCC_FLAGS = [
     "-O3",
@@ -41,30 +40,46 @@ def hipcc():
     return exe
 
 
-def _compile(sources, out, verbose):
-    srcs = [os.path.join(CSRC, s) for s in sources]
-    tmp = out + ".tmp"
-    cmd = [hipcc()] + HIPCC_FLAGS + ["-o", tmp] + srcs
+def all_library_paths():
+    """Every shared library this package loads."""
+    return [LIB_PATH]
+
+
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError(f"hipcc failed building {os.path.basename(out)}")
-    os.replace(tmp, out)
-    return out
+    return subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
 
 
-def build(verbose=False, only=None):
-    """Compile the solver library (and the warm-start conv library if present)."""
-    outs = []
-    if only in (None, "solver"):
-        outs.append(_compile(SOURCES, LIB_PATH, verbose))
-    if only in (None, "conv") and all(os.path.exists(os.path.join(CSRC, s)) for s in CONV_SOURCES):
-        outs.append(_compile(CONV_SOURCES, CONV_LIB, verbose))
-    return outs
+def build(verbose=False):
+    """Compile every source to an object (in parallel), then link the library."""
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    objs, procs = [], []
+    for s in srcs:
+        obj = os.path.join(objdir, os.path.splitext(s)[0] + ".o")
+        objs.append(obj)
+        flags = [f for f in HIPCC_FLAGS if f != "-shared"]
+        procs.append((s, _run([hipcc()] + flags + ["-c", "-o", obj, os.path.join(CSRC, s)], verbose)))
+    failed = []
+    for s, p in procs:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            sys.stderr.write(out)
+            failed.append(s)
+    if failed:
+        raise RuntimeError(f"hipcc failed on {failed}")
+    tmp = LIB_PATH + ".tmp"
+    p = _run([hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs, verbose)
+    out, _ = p.communicate()
+    if p.returncode != 0:
+        sys.stderr.write(out)
+        raise RuntimeError(f"linking {LIB_NAME} failed")
+    os.replace(tmp, LIB_PATH)
+    return [LIB_PATH]
 
 
 if __name__ == "__main__":
-    for p in build(verbose=True, only=(sys.argv[1] if len(sys.argv) > 1 else None)):
+    for p in build(verbose=True):
         print("built", p)
