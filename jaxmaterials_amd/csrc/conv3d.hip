@@ -1,0 +1,718 @@
+// Kernels of the learned warm start: a periodic 3-D convolutional
+// encoder-decoder evaluated and trained with BF16 MFMA (FP32 accumulation) on
+// gfx950.  There is no counterpart in the reference; the network predicts the
+// initial strain fluctuation from which the Lippmann-Schwinger iteration starts.
+//
+// Layouts.  Activations and gradients are bf16, channels last:
+// [instance][x][y][z][channel] with a channel stride `ld` (a multiple of 8), so
+// a decoder input that concatenates an upsampled tensor and a skip tensor is
+// one buffer whose slices are written by their producers.  Master weights are
+// fp32 [cout][27][cin] (tap = 9 (dx+1) + 3 (dy+1) + (dz+1)); the kernels read
+// packed bf16 copies:
+//   forward    Wt[co][k],  k = tap * cin + ci,   rows padded to 16, k to 32
+//   data grad  Wd[ci][k'], k' = tap' * cs + co,  Wd[ci][tap'*cs+co] = W[co][26-tap'][ci]
+// (cs = channel stride of the incoming gradient).  The data gradient is then
+// the forward convolution of the output gradient with Wd.
+//
+// MFMA.  v_mfma_f32_16x16x32_bf16, D[16x16] += A[16x32] B[32x16] with lane l
+// holding A[l%16][8(l/16)..+7], B[8(l/16)..+7][l%16] and
+// D[4(l/16)..+3][l%16].  The forward pass computes D = W^T X^T: rows are output
+// channels (A = 8 consecutive packed weights), columns are voxels (B = 8
+// consecutive channels of one neighbour voxel, one 16-byte load), so each lane
+// ends with 4 consecutive channels of one voxel (one 8-byte store).  The
+// weight gradient contracts over voxels; both operands are staged transposed
+// in LDS so a lane reads 8 consecutive voxels of one channel.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include <algorithm>
+
+typedef __attribute__((__vector_size__(8 * sizeof(short)))) short bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+
+namespace wsk {
+
+constexpr int kThreads = 256;
+constexpr float kSlope = 0.1f;  // leaky ReLU
+
+enum : int {
+  F_ACT = 1,      // leaky ReLU on the output
+  F_DACT = 2,     // multiply by the leaky ReLU derivative read from tsrc
+  F_PLANAR = 4,   // fp32 output [instance][cvalid][voxel] (+ addc[instance][c])
+};
+
+struct Geo {
+  int B, X, Y, Z;
+  int nper;  // X*Y*Z
+  int nvox;  // B*nper
+};
+
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// float -> bf16, round to nearest even (NaN stays NaN)
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ float dact(uint16_t y) { return bf2f(y) > 0.f ? 1.f : kSlope; }
+
+__device__ __forceinline__ void decompose(const Geo& g, int v, int& b, int& x, int& y, int& z) {
+  b = v / g.nper;
+  int r = v - b * g.nper;
+  const int yz = g.Y * g.Z;
+  x = r / yz;
+  r -= x * yz;
+  y = r / g.Z;
+  z = r - y * g.Z;
+}
+
+__device__ __forceinline__ int wrap(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
+
+// flat index of the periodic neighbour of (b, x, y, z) at tap
+__device__ __forceinline__ int neighbour(const Geo& g, int b, int x, int y, int z, int tap) {
+  const int dx = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dz = tap % 3 - 1;
+  return ((b * g.X + wrap(x + dx, g.X)) * g.Y + wrap(y + dy, g.Y)) * g.Z + wrap(z + dz, g.Z);
+}
+
+__device__ __forceinline__ bf16x8 load8(const uint16_t* p) { return *reinterpret_cast<const bf16x8*>(p); }
+
+// ---------------------------------------------------------------------------
+// Convolution, one wave per (CT*16 output channels) x (MT*16 voxels) tile.
+template <int CT, int MT>
+__global__ void __launch_bounds__(kThreads) k_conv(Geo g, const uint16_t* __restrict__ x, int ldx, int cin,
+                                                   const uint16_t* __restrict__ wt, int kpad,
+                                                   const float* __restrict__ bias, int cout, void* __restrict__ out,
+                                                   int ldo, int flags, const uint16_t* __restrict__ tsrc, int ldt,
+                                                   const float* __restrict__ addc, int cvalid) {
+  const int lane = threadIdx.x & 63;
+  const int wave = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+  const long long vbase = (long long)wave * (16 * MT);
+  if (vbase >= g.nvox) return;  // whole wave; no barrier in this kernel
+  const int col = lane & 15, grp = lane >> 4;
+  int vb[MT], vx[MT], vy[MT], vz[MT];
+#pragma unroll
+  for (int m = 0; m < MT; ++m) {
+    int v = (int)vbase + m * 16 + col;
+    v = v < g.nvox ? v : g.nvox - 1;  // tail lanes read a valid voxel, store nothing
+    decompose(g, v, vb[m], vx[m], vy[m], vz[m]);
+  }
+  f32x4 acc[CT][MT];
+#pragma unroll
+  for (int c = 0; c < CT; ++c)
+#pragma unroll
+    for (int m = 0; m < MT; ++m) acc[c][m] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int K = 27 * cin;
+  for (int kk = 0; kk < kpad; kk += 32) {
+    const int k0 = kk + grp * 8;
+    bf16x8 bf[MT];
+    if (k0 < K) {
+      const int tap = k0 / cin;
+      const int ci = k0 - tap * cin;
+#pragma unroll
+      for (int m = 0; m < MT; ++m) {
+        const int idx = neighbour(g, vb[m], vx[m], vy[m], vz[m], tap);
+        bf[m] = load8(x + (size_t)idx * ldx + ci);
+      }
+    } else {
+#pragma unroll
+      for (int m = 0; m < MT; ++m) bf[m] = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
+    }
+#pragma unroll
+    for (int c = 0; c < CT; ++c) {
+      const bf16x8 af = load8(wt + (size_t)(c * 16 + col) * kpad + k0);
+#pragma unroll
+      for (int m = 0; m < MT; ++m) acc[c][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bf[m], acc[c][m], 0, 0, 0);
+    }
+  }
+  // epilogue: lane holds channels co0..co0+3 of voxel v
+#pragma unroll
+  for (int m = 0; m < MT; ++m) {
+    const int v = (int)vbase + m * 16 + col;
+    if (v >= g.nvox) continue;
+#pragma unroll
+    for (int c = 0; c < CT; ++c) {
+      const int co0 = c * 16 + grp * 4;
+      if (co0 >= cout) continue;
+      float r[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        r[i] = acc[c][m][i] + (bias ? bias[co0 + i] : 0.f);
+        if (flags & F_ACT) r[i] = r[i] > 0.f ? r[i] : kSlope * r[i];
+      }
+      if (flags & F_DACT) {
+        const uint16_t* t = tsrc + (size_t)v * ldt + co0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) r[i] *= dact(t[i]);
+      }
+      if (flags & F_PLANAR) {
+        const int b = v / g.nper, p = v - b * g.nper;
+        float* o = reinterpret_cast<float*>(out);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int co = co0 + i;
+          if (co < cvalid) o[((size_t)b * cvalid + co) * g.nper + p] = r[i] + (addc ? addc[b * cvalid + co] : 0.f);
+        }
+      } else {
+        uint2 w;
+        w.x = (uint32_t)f2bf(r[0]) | ((uint32_t)f2bf(r[1]) << 16);
+        w.y = (uint32_t)f2bf(r[2]) | ((uint32_t)f2bf(r[3]) << 16);
+        *reinterpret_cast<uint2*>(reinterpret_cast<uint16_t*>(out) + (size_t)v * ldo + co0) = w;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Weight gradient partials: part[chunk][co][tap][ci] = sum over the chunk's
+// voxels v of dz[v][co] * x[neighbour(v, tap)][ci].  grid = (nchunks, 27).
+constexpr int kWgMaxCo = 128;
+constexpr int kWgMaxCi = 192;
+constexpr int kWgRow = 40;  // 32 voxels + 8 pad (bf16), 80-byte rows keep 16-byte reads aligned
+constexpr int kWgPairsPerWave = 16;
+
+__global__ void __launch_bounds__(kThreads) k_wgrad(Geo g, const uint16_t* __restrict__ dz, int ldz, int cout,
+                                                    const uint16_t* __restrict__ x, int ldx, int cin, int chunk,
+                                                    float* __restrict__ part) {
+  __shared__ __align__(16) uint16_t sdz[kWgMaxCo * kWgRow];
+  __shared__ __align__(16) uint16_t sx[kWgMaxCi * kWgRow];
+  const int tap = blockIdx.y;
+  const int c0 = blockIdx.x * chunk;
+  const int c1 = min(c0 + chunk, g.nvox);
+  const int co16 = (cout + 15) / 16, ci16 = (cin + 15) / 16;
+  const int npairs = co16 * ci16;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int col = lane & 15, grp = lane >> 4;
+  f32x4 acc[kWgPairsPerWave];
+#pragma unroll
+  for (int q = 0; q < kWgPairsPerWave; ++q) acc[q] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int ngo = co16 * 2, ngi = ci16 * 2;  // 8-channel groups (padded to 16 channels)
+  for (int v0 = c0; v0 < c1; v0 += 32) {
+    __syncthreads();
+    for (int it = threadIdx.x; it < 32 * ngo; it += kThreads) {
+      const int vl = it & 31, cg = it >> 5;
+      const int v = v0 + vl, c = cg * 8;
+      bf16x8 val = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
+      if (v < c1 && c < cout) val = load8(dz + (size_t)v * ldz + c);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) sdz[(c + j) * kWgRow + vl] = (uint16_t)val[j];
+    }
+    for (int it = threadIdx.x; it < 32 * ngi; it += kThreads) {
+      const int vl = it & 31, cg = it >> 5;
+      const int v = v0 + vl, c = cg * 8;
+      bf16x8 val = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
+      if (v < c1 && c < cin) {
+        int b, xx, yy, zz;
+        decompose(g, v, b, xx, yy, zz);
+        val = load8(x + (size_t)neighbour(g, b, xx, yy, zz, tap) * ldx + c);
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) sx[(c + j) * kWgRow + vl] = (uint16_t)val[j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kWgPairsPerWave; ++q) {
+      const int p = wave + 4 * q;
+      if (p < npairs) {
+        const int a = p / ci16, b = p - a * ci16;
+        const bf16x8 af = *reinterpret_cast<const bf16x8*>(sdz + (a * 16 + col) * kWgRow + grp * 8);
+        const bf16x8 bf = *reinterpret_cast<const bf16x8*>(sx + (b * 16 + col) * kWgRow + grp * 8);
+        acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bf, acc[q], 0, 0, 0);
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < kWgPairsPerWave; ++q) {
+    const int p = wave + 4 * q;
+    if (p < npairs) {
+      const int a = p / ci16, b = p - a * ci16;
+      const int ci = b * 16 + col;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int co = a * 16 + grp * 4 + i;
+        if (co < cout && ci < cin) part[(((size_t)blockIdx.x * cout + co) * 27 + tap) * cin + ci] = acc[q][i];
+      }
+    }
+  }
+}
+
+// out[i] = sum_c part[c][i] (fixed order)
+__global__ void __launch_bounds__(kThreads) k_sum_parts(const float* __restrict__ part, int nparts, long long len,
+                                                        float* __restrict__ out) {
+  const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= len) return;
+  float s = 0.f;
+  for (int c = 0; c < nparts; ++c) s += part[(size_t)c * len + i];
+  out[i] = s;
+}
+
+// bias gradient partials: part[block][c] = sum over the block's voxels of dz[v][c]
+__global__ void __launch_bounds__(kThreads) k_bias_parts(const uint16_t* __restrict__ dz, int ldz, int cout,
+                                                         int nvox, int chunk, float* __restrict__ part) {
+  __shared__ float red[kThreads][8];
+  const int ng = (cout + 7) / 8;      // channel groups
+  const int rows = kThreads / ng;     // voxels processed in parallel
+  const int g8 = threadIdx.x % ng, r = threadIdx.x / ng;
+  float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  const int v0 = blockIdx.x * chunk, v1 = min(v0 + chunk, nvox);
+  if (r < rows)
+    for (int v = v0 + r; v < v1; v += rows) {
+      const bf16x8 val = load8(dz + (size_t)v * ldz + g8 * 8);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) s[j] += bf2f((uint16_t)val[j]);
+    }
+#pragma unroll
+  for (int j = 0; j < 8; ++j) red[threadIdx.x][j] = (r < rows) ? s[j] : 0.f;
+  __syncthreads();
+  if (threadIdx.x < ng * 8) {
+    const int gg = threadIdx.x / 8, j = threadIdx.x % 8;
+    float t = 0.f;
+    for (int rr = 0; rr < rows; ++rr) t += red[rr * ng + gg][j];
+    const int c = gg * 8 + j;
+    if (c < cout) part[(size_t)blockIdx.x * cout + c] = t;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 2x average pooling (fine geometry g), thread per (coarse voxel, 8 channels)
+__global__ void __launch_bounds__(kThreads) k_pool(Geo g, const uint16_t* __restrict__ in, int ldi, int c,
+                                                   uint16_t* __restrict__ out, int ldo) {
+  const int ng = c / 8;
+  const int X2 = g.X / 2, Y2 = g.Y / 2, Z2 = g.Z / 2;
+  const long long n2 = (long long)g.B * X2 * Y2 * Z2;
+  const long long t = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (t >= n2 * ng) return;
+  const int cg = (int)(t % ng);
+  const long long vc = t / ng;
+  int r = (int)vc;
+  const int z = r % Z2; r /= Z2;
+  const int y = r % Y2; r /= Y2;
+  const int x = r % X2;
+  const int b = r / X2;
+  float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  for (int d = 0; d < 8; ++d) {
+    const int fi = ((b * g.X + 2 * x + (d >> 2)) * g.Y + 2 * y + ((d >> 1) & 1)) * g.Z + 2 * z + (d & 1);
+    const bf16x8 val = load8(in + (size_t)fi * ldi + cg * 8);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) s[j] += bf2f((uint16_t)val[j]);
+  }
+  bf16x8 o;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) o[j] = (short)f2bf(0.125f * s[j]);
+  *reinterpret_cast<bf16x8*>(out + (size_t)vc * ldo + cg * 8) = o;
+}
+
+// gradient of a skip tensor s (fine geometry g) that feeds both a pooling
+// and a decoder concatenation:
+//   dz[v] = act'(s[v]) * dcoarse[parent(v)] / 8 + (dcat ? dcat[v] : 0)
+// (dcat already carries act'(s), written by the decoder's data-gradient pass)
+__global__ void __launch_bounds__(kThreads) k_pool_back(Geo g, const uint16_t* __restrict__ dco, int ldc, int c,
+                                                        const uint16_t* __restrict__ s, int lds,
+                                                        const uint16_t* __restrict__ dcat, int ldcat,
+                                                        uint16_t* __restrict__ dz, int ldz) {
+  const int ng = c / 8;
+  const long long t = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (t >= (long long)g.nvox * ng) return;
+  const int cg = (int)(t % ng);
+  const int v = (int)(t / ng);
+  int b, x, y, z;
+  decompose(g, v, b, x, y, z);
+  const int X2 = g.X / 2, Y2 = g.Y / 2, Z2 = g.Z / 2;
+  const int pc = ((b * X2 + x / 2) * Y2 + y / 2) * Z2 + z / 2;
+  const bf16x8 dv = load8(dco + (size_t)pc * ldc + cg * 8);
+  const bf16x8 sv = load8(s + (size_t)v * lds + cg * 8);
+  bf16x8 cv = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
+  if (dcat) cv = load8(dcat + (size_t)v * ldcat + cg * 8);
+  bf16x8 o;
+#pragma unroll
+  for (int j = 0; j < 8; ++j)
+    o[j] = (short)f2bf(dact((uint16_t)sv[j]) * 0.125f * bf2f((uint16_t)dv[j]) + bf2f((uint16_t)cv[j]));
+  *reinterpret_cast<bf16x8*>(dz + (size_t)v * ldz + cg * 8) = o;
+}
+
+// nearest 2x upsampling into a (slice of a) fine tensor; g = fine geometry
+__global__ void __launch_bounds__(kThreads) k_upsample(Geo g, const uint16_t* __restrict__ in, int ldi, int c,
+                                                       uint16_t* __restrict__ out, int ldo) {
+  const int ng = c / 8;
+  const long long t = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (t >= (long long)g.nvox * ng) return;
+  const int cg = (int)(t % ng);
+  const int v = (int)(t / ng);
+  int b, x, y, z;
+  decompose(g, v, b, x, y, z);
+  const int pc = ((b * (g.X / 2) + x / 2) * (g.Y / 2) + y / 2) * (g.Z / 2) + z / 2;
+  *reinterpret_cast<bf16x8*>(out + (size_t)v * ldo + cg * 8) = load8(in + (size_t)pc * ldi + cg * 8);
+}
+
+// gradient of the upsampling: dcoarse[p] = sum of the 8 children; g = fine geometry
+__global__ void __launch_bounds__(kThreads) k_upsample_back(Geo g, const uint16_t* __restrict__ dfine, int ldf,
+                                                            int c, uint16_t* __restrict__ dco, int ldc) {
+  const int ng = c / 8;
+  const int X2 = g.X / 2, Y2 = g.Y / 2, Z2 = g.Z / 2;
+  const long long n2 = (long long)g.B * X2 * Y2 * Z2;
+  const long long t = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (t >= n2 * ng) return;
+  const int cg = (int)(t % ng);
+  const long long vc = t / ng;
+  int r = (int)vc;
+  const int z = r % Z2; r /= Z2;
+  const int y = r % Y2; r /= Y2;
+  const int x = r % X2;
+  const int b = r / X2;
+  float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  for (int d = 0; d < 8; ++d) {
+    const int fi = ((b * g.X + 2 * x + (d >> 2)) * g.Y + 2 * y + ((d >> 1) & 1)) * g.Z + 2 * z + (d & 1);
+    const bf16x8 val = load8(dfine + (size_t)fi * ldf + cg * 8);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) s[j] += bf2f((uint16_t)val[j]);
+  }
+  bf16x8 o;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) o[j] = (short)f2bf(s[j]);
+  *reinterpret_cast<bf16x8*>(dco + (size_t)vc * ldc + cg * 8) = o;
+}
+
+// ---------------------------------------------------------------------------
+// network input [instance][voxel][8] bf16: (log lambda - a0)/s0, (log mu - a1)/s1,
+// eps_bar[instance][0..5]; instance b uses material b / per_mat
+__global__ void __launch_bounds__(kThreads) k_input(const float* __restrict__ lam, const float* __restrict__ mu,
+                                                    int per_mat, const float* __restrict__ eps_bar, int B,
+                                                    long long nper, float a0, float s0, float a1, float s1,
+                                                    uint16_t* __restrict__ out) {
+  const long long t = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (t >= (long long)B * nper) return;
+  const int b = (int)(t / nper);
+  const long long p = t - (long long)b * nper;
+  const long long mi = (long long)(b / per_mat) * nper + p;
+  bf16x8 o;
+  o[0] = (short)f2bf((logf(lam[mi]) - a0) * s0);
+  o[1] = (short)f2bf((logf(mu[mi]) - a1) * s1);
+#pragma unroll
+  for (int j = 0; j < 6; ++j) o[2 + j] = (short)f2bf(eps_bar[b * 6 + j]);
+  *reinterpret_cast<bf16x8*>(out + (size_t)t * 8) = o;
+}
+
+// training target: bf16 planar fluctuation eps - eps_bar, [B][6][nper]
+__global__ void __launch_bounds__(kThreads) k_target(const float* __restrict__ eps, const float* __restrict__ eps_bar,
+                                                     int B, long long nper, uint16_t* __restrict__ out) {
+  const long long t = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (t >= (long long)B * 6 * nper) return;
+  const int bc = (int)(t / nper);
+  out[t] = f2bf(eps[t] - eps_bar[bc]);
+}
+
+// mean squared error of pred (fp32 planar [B][6][nper]) against target (bf16
+// planar); writes the gradient bf16 channels last [voxel][8] (channels 6, 7
+// zero) and FP64 block partial sums of the squared error
+__global__ void __launch_bounds__(kThreads) k_mse(const float* __restrict__ pred, const uint16_t* __restrict__ tgt,
+                                                  int B, long long nper, float gscale, uint16_t* __restrict__ grad,
+                                                  double* __restrict__ part) {
+  __shared__ double red[kThreads];
+  const long long v = (long long)blockIdx.x * kThreads + threadIdx.x;
+  double e = 0.0;
+  if (v < (long long)B * nper) {
+    const int b = (int)(v / nper);
+    const long long p = v - (long long)b * nper;
+    bf16x8 o = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      const size_t i = ((size_t)b * 6 + c) * nper + p;
+      const float d = pred[i] - bf2f(tgt[i]);
+      e += (double)d * d;
+      o[c] = (short)f2bf(gscale * d);
+    }
+    *reinterpret_cast<bf16x8*>(grad + (size_t)v * 8) = o;
+  }
+  red[threadIdx.x] = e;
+  __syncthreads();
+  for (int s = kThreads / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(kThreads) k_sum_f64(const double* __restrict__ part, int n, double* __restrict__ out) {
+  __shared__ double red[kThreads];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += kThreads) s += part[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = kThreads / 2; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0];
+}
+
+// Adam (Kingma & Ba 2015) on fp32 master parameters
+__global__ void __launch_bounds__(kThreads) k_adam(float* __restrict__ w, const float* __restrict__ g,
+                                                   float* __restrict__ m, float* __restrict__ v, long long n, float lr,
+                                                   float b1, float b2, float eps, float c1, float c2) {
+  const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  w[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+}
+
+// pack fp32 W[cout][27][cin] into the bf16 forward (Wt) and data-gradient (Wd) layouts
+__global__ void __launch_bounds__(kThreads) k_pack(const float* __restrict__ w, int cout, int cin, int cs,
+                                                   uint16_t* __restrict__ wt, int kpad, int rows_t,
+                                                   uint16_t* __restrict__ wd, int kpad_d, int rows_d) {
+  const long long t = (long long)blockIdx.x * kThreads + threadIdx.x;
+  const long long nt = (long long)rows_t * kpad;
+  const long long nd = wd ? (long long)rows_d * kpad_d : 0;
+  if (t < nt) {
+    const int co = (int)(t / kpad), k = (int)(t % kpad);
+    float val = 0.f;
+    if (co < cout && k < 27 * cin) {
+      const int tap = k / cin, ci = k % cin;
+      val = w[((size_t)co * 27 + tap) * cin + ci];
+    }
+    wt[t] = f2bf(val);
+  } else if (t < nt + nd) {
+    const long long u = t - nt;
+    const int ci = (int)(u / kpad_d), k = (int)(u % kpad_d);
+    float val = 0.f;
+    if (ci < cin && k < 27 * cs) {
+      const int tp = k / cs, co = k % cs;
+      if (co < cout) val = w[((size_t)co * 27 + (26 - tp)) * cin + ci];
+    }
+    wd[u] = f2bf(val);
+  }
+}
+
+}  // namespace wsk
+
+// ============================================================================
+// C ABI
+using namespace wsk;
+
+namespace {
+
+int launch_ok(const char* what) {
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    fprintf(stderr, "ls_ws: %s: %s\n", what, hipGetErrorString(e));
+    return -1;
+  }
+  return 0;
+}
+
+bool bad_geo(const int* dims) {
+  return !dims || dims[0] < 1 || dims[1] < 1 || dims[2] < 1 || dims[3] < 1 ||
+         (long long)dims[0] * dims[1] * dims[2] * dims[3] >= (1LL << 31);
+}
+
+Geo make_geo(const int* dims) {
+  Geo g;
+  g.B = dims[0];
+  g.X = dims[1];
+  g.Y = dims[2];
+  g.Z = dims[3];
+  g.nper = g.X * g.Y * g.Z;
+  g.nvox = g.B * g.nper;
+  return g;
+}
+
+unsigned nblocks(long long n) { return (unsigned)((n + kThreads - 1) / kThreads); }
+
+bool misaligned8(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) != 0; }
+
+}  // namespace
+
+extern "C" {
+
+int ls_ws_conv3d(const int* dims, const uint16_t* x, int ldx, int cin, const uint16_t* wt, int kpad,
+                 const float* bias, int cout, void* out, int ldo, int flags, const uint16_t* tsrc, int ldt,
+                 const float* addc, int cvalid, void* stream) {
+  if (bad_geo(dims) || !x || !wt || !out || cin < 8 || cin % 8 || ldx % 8 || ldx < cin || cout < 1 || cout > 192 ||
+      cout % 4 || kpad % 32 || kpad < 27 * cin || misaligned8(x) || misaligned8(wt)) {
+    fprintf(stderr, "ls_ws_conv3d: invalid arguments (cin %d ldx %d cout %d kpad %d)\n", cin, ldx, cout, kpad);
+    return -1;
+  }
+  if (!(flags & F_PLANAR) && (ldo % 4 || ldo < cout || (reinterpret_cast<uintptr_t>(out) & 7u))) {
+    fprintf(stderr, "ls_ws_conv3d: invalid output stride %d\n", ldo);
+    return -1;
+  }
+  if ((flags & F_DACT) && (!tsrc || ldt < cout)) {
+    fprintf(stderr, "ls_ws_conv3d: F_DACT needs the forward activation\n");
+    return -1;
+  }
+  if ((flags & F_PLANAR) && (cvalid < 1 || cvalid > cout)) {
+    fprintf(stderr, "ls_ws_conv3d: invalid cvalid %d\n", cvalid);
+    return -1;
+  }
+  const Geo g = make_geo(dims);
+  hipStream_t st = (hipStream_t)stream;
+  // one instantiation per tile height: the packed weights have exactly 16 * ceil(cout / 16) rows
+  const int ct = (cout + 15) / 16;
+#define LS_WS_CONV(CT, MT)                                                                                     \
+  case CT: {                                                                                                   \
+    const long long waves = ((long long)g.nvox + 16 * MT - 1) / (16 * MT);                                     \
+    k_conv<CT, MT><<<(unsigned)((waves + 3) / 4), kThreads, 0, st>>>(g, x, ldx, cin, wt, kpad, bias, cout, out,  \
+                                                                     ldo, flags, tsrc, ldt, addc, cvalid);     \
+    break;                                                                                                     \
+  }
+  switch (ct) {
+    LS_WS_CONV(1, 8)
+    LS_WS_CONV(2, 8)
+    LS_WS_CONV(3, 4)
+    LS_WS_CONV(4, 4)
+    LS_WS_CONV(6, 2)
+    LS_WS_CONV(8, 2)
+    LS_WS_CONV(12, 1)
+    default:
+      fprintf(stderr, "ls_ws_conv3d: unsupported output channel count %d\n", cout);
+      return -1;
+  }
+#undef LS_WS_CONV
+  return launch_ok("conv3d");
+}
+
+int ls_ws_conv3d_wgrad(const int* dims, const uint16_t* dz, int ldz, int cout, const uint16_t* x, int ldx, int cin,
+                       float* part, long long part_cap, float* dw, float* db, void* stream) {
+  if (bad_geo(dims) || !dz || !x || !part || !dw || cout < 1 || cout > kWgMaxCo || cin < 8 || cin > kWgMaxCi ||
+      cin % 8 || ldx % 8 || ldz % 8 || ldz < cout || ldx < cin || misaligned8(dz) || misaligned8(x)) {
+    fprintf(stderr, "ls_ws_conv3d_wgrad: invalid arguments (cout %d cin %d)\n", cout, cin);
+    return -1;
+  }
+  if (((cout + 15) / 16) * ((cin + 15) / 16) > 4 * kWgPairsPerWave) {
+    fprintf(stderr, "ls_ws_conv3d_wgrad: %d x %d channels exceed the tile budget\n", cout, cin);
+    return -1;
+  }
+  const Geo g = make_geo(dims);
+  hipStream_t st = (hipStream_t)stream;
+  const long long len = (long long)cout * 27 * cin;
+  // about 512 voxels per block, at most as many chunks as the partial buffer holds
+  int nch = (int)std::min<long long>((g.nvox + 511) / 512, 256);
+  nch = (int)std::max<long long>(1, std::min<long long>(nch, part_cap / len));
+  int chunk = (g.nvox + nch - 1) / nch;
+  chunk = (chunk + 31) / 32 * 32;
+  nch = (g.nvox + chunk - 1) / chunk;
+  if ((long long)nch * len > part_cap) {
+    fprintf(stderr, "ls_ws_conv3d_wgrad: partial buffer too small\n");
+    return -1;
+  }
+  k_wgrad<<<dim3(nch, 27), kThreads, 0, st>>>(g, dz, ldz, cout, x, ldx, cin, chunk, part);
+  k_sum_parts<<<nblocks(len), kThreads, 0, st>>>(part, nch, len, dw);
+  if (db) {
+    const int bch = 4096;
+    const int nb = (g.nvox + bch - 1) / bch;
+    if ((long long)nb * cout > part_cap) {
+      fprintf(stderr, "ls_ws_conv3d_wgrad: partial buffer too small for the bias gradient\n");
+      return -1;
+    }
+    k_bias_parts<<<nb, kThreads, 0, st>>>(dz, ldz, cout, g.nvox, bch, part);
+    k_sum_parts<<<nblocks(cout), kThreads, 0, st>>>(part, nb, cout, db);
+  }
+  return launch_ok("conv3d_wgrad");
+}
+
+int ls_ws_pool(const int* dims, const uint16_t* in, int ldi, int c, uint16_t* out, int ldo, void* stream) {
+  if (bad_geo(dims) || !in || !out || c % 8 || ldi % 8 || ldo % 8 || dims[1] % 2 || dims[2] % 2 || dims[3] % 2) {
+    fprintf(stderr, "ls_ws_pool: invalid arguments\n");
+    return -1;
+  }
+  const Geo g = make_geo(dims);
+  k_pool<<<nblocks((long long)g.nvox / 8 * (c / 8)), kThreads, 0, (hipStream_t)stream>>>(g, in, ldi, c, out, ldo);
+  return launch_ok("pool");
+}
+
+int ls_ws_pool_backward(const int* dims, const uint16_t* dcoarse, int ldc, int c, const uint16_t* s, int lds,
+                        const uint16_t* dcat, int ldcat, uint16_t* dz, int ldz, void* stream) {
+  if (bad_geo(dims) || !dcoarse || !s || !dz || c % 8 || ldc % 8 || lds % 8 || ldz % 8 || (dcat && ldcat % 8) ||
+      dims[1] % 2 || dims[2] % 2 || dims[3] % 2) {
+    fprintf(stderr, "ls_ws_pool_backward: invalid arguments\n");
+    return -1;
+  }
+  const Geo g = make_geo(dims);
+  k_pool_back<<<nblocks((long long)g.nvox * (c / 8)), kThreads, 0, (hipStream_t)stream>>>(g, dcoarse, ldc, c, s, lds,
+                                                                                           dcat, ldcat, dz, ldz);
+  return launch_ok("pool_backward");
+}
+
+int ls_ws_upsample(const int* dims, const uint16_t* in, int ldi, int c, uint16_t* out, int ldo, void* stream) {
+  if (bad_geo(dims) || !in || !out || c % 8 || ldi % 8 || ldo % 8 || dims[1] % 2 || dims[2] % 2 || dims[3] % 2) {
+    fprintf(stderr, "ls_ws_upsample: invalid arguments\n");
+    return -1;
+  }
+  const Geo g = make_geo(dims);
+  k_upsample<<<nblocks((long long)g.nvox * (c / 8)), kThreads, 0, (hipStream_t)stream>>>(g, in, ldi, c, out, ldo);
+  return launch_ok("upsample");
+}
+
+int ls_ws_upsample_backward(const int* dims, const uint16_t* dfine, int ldf, int c, uint16_t* dcoarse, int ldc,
+                            void* stream) {
+  if (bad_geo(dims) || !dfine || !dcoarse || c % 8 || ldf % 8 || ldc % 8 || dims[1] % 2 || dims[2] % 2 ||
+      dims[3] % 2) {
+    fprintf(stderr, "ls_ws_upsample_backward: invalid arguments\n");
+    return -1;
+  }
+  const Geo g = make_geo(dims);
+  k_upsample_back<<<nblocks((long long)g.nvox / 8 * (c / 8)), kThreads, 0, (hipStream_t)stream>>>(g, dfine, ldf, c,
+                                                                                                   dcoarse, ldc);
+  return launch_ok("upsample_backward");
+}
+
+int ls_ws_input(const float* lam, const float* mu, int per_mat, const float* eps_bar, int B, long long nper,
+                const float* norm, uint16_t* out, void* stream) {
+  if (!lam || !mu || !eps_bar || !norm || !out || per_mat < 1 || B < 1 || nper < 1) {
+    fprintf(stderr, "ls_ws_input: invalid arguments\n");
+    return -1;
+  }
+  k_input<<<nblocks((long long)B * nper), kThreads, 0, (hipStream_t)stream>>>(lam, mu, per_mat, eps_bar, B, nper,
+                                                                               norm[0], norm[1], norm[2], norm[3], out);
+  return launch_ok("input");
+}
+
+int ls_ws_target(const float* eps, const float* eps_bar, int B, long long nper, uint16_t* out, void* stream) {
+  if (!eps || !eps_bar || !out || B < 1 || nper < 1) return -1;
+  k_target<<<nblocks((long long)B * 6 * nper), kThreads, 0, (hipStream_t)stream>>>(eps, eps_bar, B, nper, out);
+  return launch_ok("target");
+}
+
+int ls_ws_mse(const float* pred, const uint16_t* target, int B, long long nper, float gscale, uint16_t* grad,
+              double* part, long long part_cap, double* loss, void* stream) {
+  if (!pred || !target || !grad || !part || !loss || B < 1 || nper < 1) return -1;
+  const long long nv = (long long)B * nper;
+  const unsigned nb = nblocks(nv);
+  if ((long long)nb > part_cap) {
+    fprintf(stderr, "ls_ws_mse: partial buffer too small\n");
+    return -1;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  k_mse<<<nb, kThreads, 0, st>>>(pred, target, B, nper, gscale, grad, part);
+  k_sum_f64<<<1, kThreads, 0, st>>>(part, (int)nb, loss);
+  return launch_ok("mse");
+}
+
+int ls_ws_adam(float* w, const float* g, float* m, float* v, long long n, float lr, float b1, float b2, float eps,
+               int step, void* stream) {
+  if (!w || !g || !m || !v || n < 1 || step < 1) return -1;
+  const float c1 = 1.f / (1.f - powf(b1, (float)step));
+  const float c2 = 1.f / (1.f - powf(b2, (float)step));
+  k_adam<<<nblocks(n), kThreads, 0, (hipStream_t)stream>>>(w, g, m, v, n, lr, b1, b2, eps, c1, c2);
+  return launch_ok("adam");
+}
+
+int ls_ws_pack(const float* w, int cout, int cin, int cs, uint16_t* wt, int kpad, int rows_t, uint16_t* wd,
+               int kpad_d, int rows_d, void* stream) {
+  if (!w || !wt || cout < 1 || cin < 1 || kpad < 27 * cin || rows_t < cout || (wd && (kpad_d < 27 * cs || cs < cout ||
+                                                                                        rows_d < cin))) {
+    fprintf(stderr, "ls_ws_pack: invalid arguments\n");
+    return -1;
+  }
+  const long long n = (long long)rows_t * kpad + (wd ? (long long)rows_d * kpad_d : 0);
+  k_pack<<<nblocks(n), kThreads, 0, (hipStream_t)stream>>>(w, cout, cin, cs, wt, kpad, rows_t, wd, kpad_d, rows_d);
+  return launch_ok("pack");
+}
+
+}  // extern "C"

@@ -1,0 +1,138 @@
+"""GPU parity of the warm-start network kernels (csrc/conv3d.hip, through the C
+ABI) against the CPU FP64 oracle of oracle/warmstart_oracle.py, which rounds
+to bf16 where the GPU stores bf16.
+
+Tolerances: forward output within 1e-2 relative L2 (bf16 storage of 15
+activations, FP32 accumulation); loss within 1e-3 relative; every layer's
+weight and bias gradient within 3e-2 relative L2 (the GPU also stores the
+backpropagated gradients in bf16, the oracle keeps them in FP64).
+"""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _bf16_bits(a):
+    """float32 array -> bf16 bit patterns (round to nearest even) as int16."""
+    u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+    return r.view(np.int16)
+
+
+def _setup(gpu, n=16, nmat=2, per_mat=2, seed=3):
+    import torch
+
+    from jaxmaterials_amd.warmstart.net import WarmStartNet
+
+    rng = np.random.default_rng(seed)
+    net = WarmStartNet(device=gpu, seed=seed)
+    # random nonzero biases so that the bias paths are exercised
+    sd = net.state_dict()
+    for k, (w, b) in sd.items():
+        b[:] = rng.uniform(-0.05, 0.05, b.shape)
+        if k == "head":
+            b[6:] = 0.0
+    net.load_state_dict(sd)
+    lam = rng.choice([2.593, 19.44], size=(nmat, n, n, n)).astype(np.float32)
+    mu = np.where(lam > 3.0, 29.17, 1.111).astype(np.float32)
+    B = nmat * per_mat
+    eb = rng.standard_normal((B, 6)).astype(np.float32) * 0.5
+    target = (rng.standard_normal((B, 6, n, n, n)) * 0.2).astype(np.float32)
+    dev = dict(device=gpu)
+    t = dict(
+        lam=torch.tensor(lam, **dev), mu=torch.tensor(mu, **dev), eb=torch.tensor(eb, **dev),
+        target=torch.tensor(_bf16_bits(target), **dev),
+    )
+    return net, lam, mu, eb, target, t, B, per_mat
+
+
+def _oracle(net, lam, mu, eb, per_mat, target):
+    from oracle import warmstart_oracle as W
+
+    sd = {k: (w.astype(np.float64), b.astype(np.float64)) for k, (w, b) in net.state_dict().items()}
+    return W.loss_and_grads(sd, lam.astype(np.float64), mu.astype(np.float64), eb.astype(np.float64), per_mat,
+                            net.norm, target.astype(np.float64), bf16=True)
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def test_forward_matches_oracle(gpu):
+    import torch
+
+    net, lam, mu, eb, target, t, B, per_mat = _setup(gpu)
+    n = lam.shape[-1]
+    out = torch.empty((B, 6, n, n, n), dtype=torch.float32, device=gpu)
+    net.forward(t["lam"], t["mu"], t["eb"], per_mat, out, add_eps_bar=True)
+    torch.cuda.synchronize()
+    loss, pred, _ = _oracle(net, lam, mu, eb, per_mat, target)
+    got = out.cpu().numpy()
+    want = pred + eb[:, :, None, None, None]
+    err = _rel(got - eb[:, :, None, None, None], pred)
+    print(f"forward: rel L2 err of the fluctuation {err:.2e}")
+    assert err < 1e-2
+    assert _rel(got, want) < 1e-2
+
+
+def test_gradients_match_oracle(gpu):
+    import torch
+
+    net, lam, mu, eb, target, t, B, per_mat = _setup(gpu, seed=5)
+    n = lam.shape[-1]
+    pred = torch.empty((B, 6, n, n, n), dtype=torch.float32, device=gpu)
+    net.forward(t["lam"], t["mu"], t["eb"], per_mat, pred, add_eps_bar=False)
+    loss_dev = net.loss_backward(pred, t["target"])
+    torch.cuda.synchronize()
+    loss_gpu = float(loss_dev.item()) / (B * 6 * n**3)
+    loss, _, grads = _oracle(net, lam, mu, eb, per_mat, target)
+    print(f"loss gpu {loss_gpu:.6e} oracle {loss:.6e}")
+    assert abs(loss_gpu - loss) <= 1e-3 * loss
+    g = net.grads_dict()
+    worst = 0.0
+    for name, (dw, db) in grads.items():
+        gw, gb = g[name]
+        if name == "head":
+            gw, gb, dw, db = gw[:6], gb[:6], dw[:6], db[:6]
+            assert np.all(g[name][0][6:] == 0) and np.all(g[name][1][6:] == 0)
+        ew, eb_ = _rel(gw, dw), _rel(gb, db)
+        cos = float(np.dot(gw.ravel(), dw.ravel()) / (np.linalg.norm(gw) * np.linalg.norm(dw)))
+        print(f"{name:6s} dW rel {ew:.2e} cos {cos:.6f}  db rel {eb_:.2e}")
+        worst = max(worst, ew, eb_)
+        assert ew < 3e-2 and eb_ < 3e-2 and cos > 0.999, name
+    print(f"worst gradient rel L2 err {worst:.2e}")
+
+
+def test_adam_reduces_loss(gpu):
+    """Adam steps on one batch lower its loss toward a learnable target (zero
+    fluctuation): the optimiser and the repacking of updated weights work end
+    to end."""
+    import torch
+
+    net, lam, mu, eb, target, t, B, per_mat = _setup(gpu, seed=7)
+    n = lam.shape[-1]
+    zero = torch.zeros_like(t["target"])
+    pred = torch.empty((B, 6, n, n, n), dtype=torch.float32, device=gpu)
+    losses = []
+    for _ in range(12):
+        net.forward(t["lam"], t["mu"], t["eb"], per_mat, pred, add_eps_bar=False)
+        losses.append(float(net.loss_backward(pred, zero).item()))
+        net.adam_step_(1e-4)
+    torch.cuda.synchronize()
+    print("losses", [f"{v:.4e}" for v in losses])
+    assert losses[-1] < 0.5 * losses[0]
+
+
+def test_invalid_arguments_are_rejected(gpu):
+    import ctypes
+
+    from jaxmaterials_amd import _lib
+
+    lib = _lib.load()
+    dims = (ctypes.c_int * 4)(1, 8, 8, 8)
+    # cin not a multiple of 8: refused before any launch
+    rc = lib.ls_ws_conv3d(dims, ctypes.c_void_p(16), 8, 6, ctypes.c_void_p(16), 192, None, 16, ctypes.c_void_p(16),
+                          16, 0, None, 0, None, 0, None)
+    assert rc == -1
