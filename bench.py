@@ -1,12 +1,14 @@
 #!/usr/bin/env python
-"""C_eff solves/sec to a 1e-6 relative residual at 256^3 (plain LS vs batched Anderson).
+"""C_eff solves/sec to a 1e-6 relative residual at 256^3: plain LS, batched
+Anderson, and Anderson from the learned warm start.
 
 One step = one effective-tensor solve: the six unit Voigt load cases of a
 256^3 cross-ply microstructure (16 layers of 16 voxels, fibre radius 6 voxels,
 RSA to volume fraction 0.5, matrix (lambda, mu) = (2.593, 1.111), fibre
 (19.44, 29.17)) batched in one solver call, iterated to R <= 1e-6 R(eps_bar),
 then the FP64 volume averages that form the 6x6 C_eff.  Inputs are generated
-on the GPU and resident in HBM before the timed region.
+on the GPU and resident in HBM before the timed region.  The warm-started
+method ("aa_ws") times the network inference and the solve together.
 
 Prints one JSON line.  Usage: python bench.py [--steps K] [--warmup W]
 """
@@ -117,7 +119,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--grid", type=int, default=256)
     ap.add_argument("--depth", type=int, default=5)
-    ap.add_argument("--methods", default="plain,aa")
+    ap.add_argument("--methods", default="plain,aa,aa_ws")
     ap.add_argument("--rtol", type=float, default=1e-6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -152,12 +154,37 @@ def main():
             golden = np.array(json.load(f)["ceff"])
 
     results = {}
+    net = None
     for method in args.methods.split(","):
-        depth = 0 if method == "plain" else args.depth
+        depth = 0 if method.startswith("plain") else args.depth
+        warm = method.endswith("_ws")
         solver = LSSolver(gs, nrhs=6, depth=depth)
         loads = unit_load_cases()
+        eps = torch.empty((6, 6) + shape, dtype=torch.float32, device="cuda")
+        ws_info = None
+        if warm:
+            from jaxmaterials_amd.warmstart import WarmStartNet
+            from jaxmaterials_amd.warmstart.train import DEFAULT_WEIGHTS, warm_start_solve
+            if net is None:
+                net = WarmStartNet.load(DEFAULT_WEIGHTS)
+            r_init = warm_start_solve(net, solver, lam, mu, loads, epsilon=eps, rtol=args.rtol, maxiter=0,
+                                      check=False)["residual"]
+            eb6 = torch.eye(6, device="cuda")
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            net.forward(lam.reshape((1,) + shape), mu.reshape((1,) + shape), eb6, 6, eps)
+            torch.cuda.synchronize()
+            ws_info = {"initial_residual_vs_uniform": [float(v) for v in r_init],
+                       "inference_ms": 1e3 * (time.perf_counter() - t0),
+                       "weights": os.path.basename(DEFAULT_WEIGHTS)}
+
+        def run():
+            if warm:
+                return warm_start_solve(net, solver, lam, mu, loads, epsilon=eps, rtol=args.rtol, maxiter=3000)
+            return solver.solve(lam, mu, loads, epsilon=eps, rtol=args.rtol, maxiter=3000)
+
         for _ in range(args.warmup):
-            solver.solve(lam, mu, loads, rtol=args.rtol, maxiter=3000)
+            run()
         torch.cuda.synchronize()
         times = []
         ms_kind = {k: 0.0 for k in LSSolver.KINDS}
@@ -167,7 +194,7 @@ def main():
         for _ in range(args.steps):
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            out = solver.solve(lam, mu, loads, rtol=args.rtol, maxiter=3000)
+            out = run()
             torch.cuda.synchronize()
             times.append(time.perf_counter() - t0)
             ms, cnt, nreb = solver.last_timing()
@@ -191,6 +218,8 @@ def main():
         rec = {"solves_per_s": 1.0 / sec, "ms_per_solve": sec * 1e3, "iters": iters, "depth": depth,
                "base_reevaluations": nreb, "passes": kinds, "loop_GBps": loop_bytes / sec / 1e9,
                "ceff": C.tolist()}
+        if ws_info is not None:
+            rec["warm_start"] = ws_info
         if golden is not None:
             sig = np.abs(golden) > 1e-3 * np.abs(golden).max()
             rec["ceff_max_rel_err_vs_fp64_oracle"] = float((np.abs(C - golden)[sig] / np.abs(golden)[sig]).max())
@@ -199,7 +228,9 @@ def main():
               + ", ".join(f"{k} {v['achieved_GBps']:.0f} GB/s ({v['ms_total']:.0f} ms)" for k, v in kinds.items()
                           if v['achieved_GBps']), file=sys.stderr, flush=True)
         solver.close()
-        del solver
+        del solver, eps
+        if net is not None:
+            net.release()
         torch.cuda.empty_cache()
 
     best_name = max(results, key=lambda k: results[k]["solves_per_s"])
@@ -209,7 +240,7 @@ def main():
     peak = 8000.0
     roofline = {"bound": "hbm", "kernel": dk["kernel"], "achieved": dk["achieved_GBps"], "peak": peak,
                 "unit": "GB/s", "frac": dk["achieved_GBps"] / peak,
-                "traffic": measured_traffic(dom, n, best_name),
+                "traffic": measured_traffic(dom, n, best_name.replace("_ws", "")),
                 "algorithmic_bytes_per_launch": dk["algorithmic_bytes"] / max(1, dk["launches"]),
                 "avg_launch_ms": dk["avg_launch_ms"],
                 "loop_achieved": best["loop_GBps"], "loop_frac": best["loop_GBps"] / peak}
@@ -238,7 +269,8 @@ def main():
         "methods": {k: {kk: vv for kk, vv in v.items() if kk != "ceff"} for k, v in results.items()},
     }
     if "plain" in results and len(results) > 1:
-        line["speedup_vs_plain"] = best["solves_per_s"] / results["plain"]["solves_per_s"]
+        line["speedup_vs_plain"] = {k: v["solves_per_s"] / results["plain"]["solves_per_s"]
+                                    for k, v in results.items() if k != "plain"}
     print(json.dumps(line))
 
 

@@ -41,10 +41,14 @@ int ls_ws_conv3d(const int *dims, const uint16_t *x, int ldx, int cin,
 
 /* Weight and bias gradients of one convolution: dw[cout][27][cin] =
  * sum_v dz[v][co] x[v + tap][ci], db[cout] = sum_v dz[v][co] (db may be NULL).
+ * cout, cin multiples of 8, the voxel count a multiple of 8.  planar =
+ * bf16 scratch of
+ * planar_cap >= (cout + cin) * voxels elements (planar copies of dz and x);
  * part = fp32 scratch of part_cap floats (deterministic two-stage sums). */
 int ls_ws_conv3d_wgrad(const int *dims, const uint16_t *dz, int ldz, int cout,
-                       const uint16_t *x, int ldx, int cin, float *part,
-                       long long part_cap, float *dw, float *db, void *stream);
+                       const uint16_t *x, int ldx, int cin, uint16_t *planar,
+                       long long planar_cap, float *part, long long part_cap,
+                       float *dw, float *db, void *stream);
 
 /* 2x average pooling; dims = the fine grid (even sizes). */
 int ls_ws_pool(const int *dims, const uint16_t *in, int ldi, int c,
@@ -88,8 +92,10 @@ int ls_ws_adam(float *w, const float *g, float *m, float *v, long long n,
                float lr, float b1, float b2, float eps, int step, void *stream);
 
 /* Pack fp32 W[cout][27][cin] into the bf16 forward layout wt (rows_t rows
- * of kpad) and, if wd != NULL, the data-gradient layout wd (rows_d rows of
- * kpad_d = roundup(27*cs, 32), cs = channel stride of the output gradient). */
+ * of kpad = roundup(27*cin, 32), K index tap*cin + ci) and, if wd != NULL,
+ * the data-gradient layout wd (rows_d rows of kpad_d = roundup(27*cs, 32),
+ * K index tap'*cs + co with the tap mirrored, cs = channel stride of the
+ * output gradient). */
 int ls_ws_pack(const float *w, int cout, int cin, int cs, uint16_t *wt,
                int kpad, int rows_t, uint16_t *wd, int kpad_d, int rows_d,
                void *stream);

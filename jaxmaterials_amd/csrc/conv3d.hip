@@ -20,8 +20,8 @@
 // channels (A = 8 consecutive packed weights), columns are voxels (B = 8
 // consecutive channels of one neighbour voxel, one 16-byte load), so each lane
 // ends with 4 consecutive channels of one voxel (one 8-byte store).  The
-// weight gradient contracts over voxels; both operands are staged transposed
-// in LDS so a lane reads 8 consecutive voxels of one channel.
+// weight gradient contracts over voxels; both operands are copied to planar
+// layouts so a lane reads 8 consecutive voxels of one channel.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -167,75 +167,118 @@ __global__ void __launch_bounds__(kThreads) k_conv(Geo g, const uint16_t* __rest
 }
 
 // ---------------------------------------------------------------------------
-// Weight gradient partials: part[chunk][co][tap][ci] = sum over the chunk's
-// voxels v of dz[v][co] * x[neighbour(v, tap)][ci].  grid = (nchunks, 27).
+// Weight gradient.  Both operands are first copied to planar bf16 [c][nvox]
+// (k_planar), so a lane reads 8 consecutive voxels of one channel with one
+// 16-byte load; the z offset of a tap is applied by shifting the 8 values by
+// one element across two loads (z runs are multiples of 8, periodic).
+// part[chunk][co][tap][ci] = sum over the chunk's voxels v of
+// dz[v][co] * x[neighbour(v, tap)][ci]; one wave per (chunk, tap, 16-channel
+// input tile), all output tiles of the wave accumulated in registers.
 constexpr int kWgMaxCo = 128;
 constexpr int kWgMaxCi = 192;
-constexpr int kWgRow = 40;  // 32 voxels + 8 pad (bf16), 80-byte rows keep 16-byte reads aligned
-constexpr int kWgPairsPerWave = 16;
 
-__global__ void __launch_bounds__(kThreads) k_wgrad(Geo g, const uint16_t* __restrict__ dz, int ldz, int cout,
-                                                    const uint16_t* __restrict__ x, int ldx, int cin, int chunk,
-                                                    float* __restrict__ part) {
-  __shared__ __align__(16) uint16_t sdz[kWgMaxCo * kWgRow];
-  __shared__ __align__(16) uint16_t sx[kWgMaxCi * kWgRow];
-  const int tap = blockIdx.y;
-  const int c0 = blockIdx.x * chunk;
-  const int c1 = min(c0 + chunk, g.nvox);
-  const int co16 = (cout + 15) / 16, ci16 = (cin + 15) / 16;
-  const int npairs = co16 * ci16;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+// channels-last [nvox][ld] (first c channels) -> planar [c][nvox]
+__global__ void __launch_bounds__(kThreads) k_planar(const uint16_t* __restrict__ in, int ld, int c, int nvox,
+                                                     uint16_t* __restrict__ out) {
+  const int ng = c / 8;
+  const long long t = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (t >= (long long)(nvox + 63) / 64 * 64 * ng) return;
+  // consecutive threads take consecutive voxels of one channel group, so the
+  // 2-byte planar stores of a wave are contiguous
+  const long long blk = t / (64LL * ng);
+  const int r = (int)(t - blk * 64LL * ng);
+  const int cg = r / 64;
+  const long long v = blk * 64 + (r % 64);
+  if (v >= nvox) return;
+  const bf16x8 val = load8(in + (size_t)v * ld + cg * 8);
+#pragma unroll
+  for (int j = 0; j < 8; ++j) out[(size_t)(cg * 8 + j) * nvox + v] = (uint16_t)val[j];
+}
+
+__device__ __forceinline__ bf16x8 shift_in_next(bf16x8 a, bf16x8 b) {
+  // elements 1..7 of a, then element 0 of b
+  bf16x8 o;
+#pragma unroll
+  for (int j = 0; j < 7; ++j) o[j] = a[j + 1];
+  o[7] = b[0];
+  return o;
+}
+
+__device__ __forceinline__ bf16x8 shift_in_prev(bf16x8 p, bf16x8 a) {
+  // element 7 of p, then elements 0..6 of a
+  bf16x8 o;
+  o[0] = p[7];
+#pragma unroll
+  for (int j = 1; j < 8; ++j) o[j] = a[j - 1];
+  return o;
+}
+
+template <int COT, bool ZRUN8>
+__global__ void __launch_bounds__(kThreads) k_wgrad(Geo g, const uint16_t* __restrict__ dzp, int cout,
+                                                    const uint16_t* __restrict__ xp, int cin, int ci16, int chunk,
+                                                    int nwork, float* __restrict__ part) {
+  const int lane = threadIdx.x & 63;
+  const int w = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+  if (w >= nwork) return;  // whole wave; no barrier in this kernel
   const int col = lane & 15, grp = lane >> 4;
-  f32x4 acc[kWgPairsPerWave];
+  const int bt = w % ci16;
+  const int tap = (w / ci16) % 27;
+  const int ch = w / (ci16 * 27);
+  const int dx = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dz = tap % 3 - 1;
+  const int v0 = ch * chunk, v1 = min(v0 + chunk, g.nvox);
+  const int ci = bt * 16 + col;
+  const bool ci_ok = ci < cin;
+  const uint16_t* xrow = xp + (size_t)(ci_ok ? ci : 0) * g.nvox;
+  f32x4 acc[COT];
 #pragma unroll
-  for (int q = 0; q < kWgPairsPerWave; ++q) acc[q] = f32x4{0.f, 0.f, 0.f, 0.f};
-  const int ngo = co16 * 2, ngi = ci16 * 2;  // 8-channel groups (padded to 16 channels)
-  for (int v0 = c0; v0 < c1; v0 += 32) {
-    __syncthreads();
-    for (int it = threadIdx.x; it < 32 * ngo; it += kThreads) {
-      const int vl = it & 31, cg = it >> 5;
-      const int v = v0 + vl, c = cg * 8;
-      bf16x8 val = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
-      if (v < c1 && c < cout) val = load8(dz + (size_t)v * ldz + c);
+  for (int a = 0; a < COT; ++a) acc[a] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const bf16x8 zero8 = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
+  for (int vb = v0; vb < v1; vb += 32) {
+    const int vs = vb + grp * 8;  // this lane's 8 voxels (nvox % 8 == 0, so all or none are valid)
+    const bool v_ok = vs < v1;
+    bf16x8 bv = zero8;
+    if (ci_ok && v_ok) {
+      if (ZRUN8) {
+        // the 8 voxels are one segment of a z run (Z % 8 == 0)
+        int b, x, y, z;
+        decompose(g, vs, b, x, y, z);
+        const size_t rowb = (size_t)((b * g.X + wrap(x + dx, g.X)) * g.Y + wrap(y + dy, g.Y)) * g.Z;
+        const bf16x8 c0 = load8(xrow + rowb + z);
+        if (dz == 0) {
+          bv = c0;
+        } else if (dz > 0) {
+          const int zn = z + 8 == g.Z ? 0 : z + 8;
+          bv = shift_in_next(c0, load8(xrow + rowb + zn));
+        } else {
+          const int zp = z == 0 ? g.Z - 8 : z - 8;
+          bv = shift_in_prev(load8(xrow + rowb + zp), c0);
+        }
+      } else {
+        // short z runs (coarse levels): one neighbour lookup per voxel
 #pragma unroll
-      for (int j = 0; j < 8; ++j) sdz[(c + j) * kWgRow + vl] = (uint16_t)val[j];
-    }
-    for (int it = threadIdx.x; it < 32 * ngi; it += kThreads) {
-      const int vl = it & 31, cg = it >> 5;
-      const int v = v0 + vl, c = cg * 8;
-      bf16x8 val = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
-      if (v < c1 && c < cin) {
-        int b, xx, yy, zz;
-        decompose(g, v, b, xx, yy, zz);
-        val = load8(x + (size_t)neighbour(g, b, xx, yy, zz, tap) * ldx + c);
+        for (int j = 0; j < 8; ++j) {
+          int b, x, y, z;
+          decompose(g, vs + j, b, x, y, z);
+          bv[j] = (short)xrow[neighbour(g, b, x, y, z, tap)];
+        }
       }
-#pragma unroll
-      for (int j = 0; j < 8; ++j) sx[(c + j) * kWgRow + vl] = (uint16_t)val[j];
     }
-    __syncthreads();
 #pragma unroll
-    for (int q = 0; q < kWgPairsPerWave; ++q) {
-      const int p = wave + 4 * q;
-      if (p < npairs) {
-        const int a = p / ci16, b = p - a * ci16;
-        const bf16x8 af = *reinterpret_cast<const bf16x8*>(sdz + (a * 16 + col) * kWgRow + grp * 8);
-        const bf16x8 bf = *reinterpret_cast<const bf16x8*>(sx + (b * 16 + col) * kWgRow + grp * 8);
-        acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bf, acc[q], 0, 0, 0);
-      }
+    for (int a = 0; a < COT; ++a) {
+      const int co = a * 16 + col;
+      const bf16x8 av = (co < cout && v_ok) ? load8(dzp + (size_t)co * g.nvox + vs) : zero8;
+      acc[a] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[a], 0, 0, 0);
     }
   }
+  // D rows = output channels a*16 + 4 grp + i, columns = input channel ci
+  if (ci_ok) {
 #pragma unroll
-  for (int q = 0; q < kWgPairsPerWave; ++q) {
-    const int p = wave + 4 * q;
-    if (p < npairs) {
-      const int a = p / ci16, b = p - a * ci16;
-      const int ci = b * 16 + col;
+    for (int a = 0; a < COT; ++a)
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int co = a * 16 + grp * 4 + i;
-        if (co < cout && ci < cin) part[(((size_t)blockIdx.x * cout + co) * 27 + tap) * cin + ci] = acc[q][i];
+        if (co < cout) part[(((size_t)ch * cout + co) * 27 + tap) * cin + ci] = acc[a][i];
       }
-    }
   }
 }
 
@@ -578,31 +621,60 @@ int ls_ws_conv3d(const int* dims, const uint16_t* x, int ldx, int cin, const uin
 }
 
 int ls_ws_conv3d_wgrad(const int* dims, const uint16_t* dz, int ldz, int cout, const uint16_t* x, int ldx, int cin,
-                       float* part, long long part_cap, float* dw, float* db, void* stream) {
-  if (bad_geo(dims) || !dz || !x || !part || !dw || cout < 1 || cout > kWgMaxCo || cin < 8 || cin > kWgMaxCi ||
-      cin % 8 || ldx % 8 || ldz % 8 || ldz < cout || ldx < cin || misaligned8(dz) || misaligned8(x)) {
-    fprintf(stderr, "ls_ws_conv3d_wgrad: invalid arguments (cout %d cin %d)\n", cout, cin);
-    return -1;
-  }
-  if (((cout + 15) / 16) * ((cin + 15) / 16) > 4 * kWgPairsPerWave) {
-    fprintf(stderr, "ls_ws_conv3d_wgrad: %d x %d channels exceed the tile budget\n", cout, cin);
+                       uint16_t* planar, long long planar_cap, float* part, long long part_cap, float* dw, float* db,
+                       void* stream) {
+  if (bad_geo(dims) || !dz || !x || !planar || !part || !dw || cout < 1 || cout > kWgMaxCo || cout % 8 || cin < 8 ||
+      cin > kWgMaxCi || cin % 8 || ldx % 8 || ldz % 8 || ldz < cout || ldx < cin || misaligned8(dz) ||
+      misaligned8(x) || misaligned8(planar)) {
+    fprintf(stderr, "ls_ws_conv3d_wgrad: invalid arguments (cout %d cin %d nz %d)\n", cout, cin, dims ? dims[3] : 0);
     return -1;
   }
   const Geo g = make_geo(dims);
+  if (g.nvox % 8) {
+    fprintf(stderr, "ls_ws_conv3d_wgrad: the voxel count must be a multiple of 8\n");
+    return -1;
+  }
+  if ((long long)(cout + cin) * g.nvox > planar_cap) {
+    fprintf(stderr, "ls_ws_conv3d_wgrad: planar buffer too small\n");
+    return -1;
+  }
   hipStream_t st = (hipStream_t)stream;
+  uint16_t* dzp = planar;
+  uint16_t* xp = planar + (size_t)cout * g.nvox;
+  const long long nv64 = (g.nvox + 63) / 64 * 64;
+  k_planar<<<nblocks(nv64 * (cout / 8)), kThreads, 0, st>>>(dz, ldz, cout, g.nvox, dzp);
+  k_planar<<<nblocks(nv64 * (cin / 8)), kThreads, 0, st>>>(x, ldx, cin, g.nvox, xp);
   const long long len = (long long)cout * 27 * cin;
-  // about 512 voxels per block, at most as many chunks as the partial buffer holds
-  int nch = (int)std::min<long long>((g.nvox + 511) / 512, 256);
-  nch = (int)std::max<long long>(1, std::min<long long>(nch, part_cap / len));
-  int chunk = (g.nvox + nch - 1) / nch;
-  chunk = (chunk + 31) / 32 * 32;
+  const int ci16 = (cin + 15) / 16;
+  // about 8192 waves, chunks of whole 32-voxel steps, as many as the partial buffer holds
+  const int steps = (g.nvox + 31) / 32;
+  long long nch = std::max<long long>(1, std::min<long long>(8192 / (27 * ci16) + 1, steps));
+  nch = std::max<long long>(1, std::min<long long>(nch, part_cap / len));
+  const int chunk = (int)((steps + nch - 1) / nch) * 32;
   nch = (g.nvox + chunk - 1) / chunk;
-  if ((long long)nch * len > part_cap) {
+  if (nch * len > part_cap) {
     fprintf(stderr, "ls_ws_conv3d_wgrad: partial buffer too small\n");
     return -1;
   }
-  k_wgrad<<<dim3(nch, 27), kThreads, 0, st>>>(g, dz, ldz, cout, x, ldx, cin, chunk, part);
-  k_sum_parts<<<nblocks(len), kThreads, 0, st>>>(part, nch, len, dw);
+  const int nwork = (int)(nch * 27 * ci16);
+  const unsigned nblk = (unsigned)((nwork + 3) / 4);
+  const int cot = (cout + 15) / 16;
+#define LS_WS_WGRAD(COT)                                                                                    \
+  if (g.Z % 8 == 0)                                                                                         \
+    k_wgrad<COT, true><<<nblk, kThreads, 0, st>>>(g, dzp, cout, xp, cin, ci16, chunk, nwork, part);         \
+  else                                                                                                      \
+    k_wgrad<COT, false><<<nblk, kThreads, 0, st>>>(g, dzp, cout, xp, cin, ci16, chunk, nwork, part);
+  if (cot == 1) {
+    LS_WS_WGRAD(1)
+  } else if (cot == 2) {
+    LS_WS_WGRAD(2)
+  } else if (cot <= 4) {
+    LS_WS_WGRAD(4)
+  } else {
+    LS_WS_WGRAD(8)
+  }
+#undef LS_WS_WGRAD
+  k_sum_parts<<<nblocks(len), kThreads, 0, st>>>(part, (int)nch, len, dw);
   if (db) {
     const int bch = 4096;
     const int nb = (g.nvox + bch - 1) / bch;

@@ -202,9 +202,12 @@ class WarmStartNet:
 
     def _wgrad(self, L, dims, dz, x, ldx):
         dims_a = (ctypes.c_int * 4)(*dims)
-        _check(self.lib.ls_ws_conv3d_wgrad(dims_a, _p(dz), L.cs, L.cout, _p(x), ldx, L.cin, _p(self.part),
-                                           self.part.numel(), _p(self.grads[L.woff:]), _p(self.grads[L.boff:]),
-                                           self.stream()), L.name + " wgrad")
+        nvox = dims[0] * dims[1] * dims[2] * dims[3]
+        planar = self._buf("planar", nvox, L.cout + L.cin)
+        _check(self.lib.ls_ws_conv3d_wgrad(dims_a, _p(dz), L.cs, L.cout, _p(x), ldx, L.cin, _p(planar),
+                                           planar.numel(), _p(self.part), self.part.numel(),
+                                           _p(self.grads[L.woff:]), _p(self.grads[L.boff:]), self.stream()),
+               L.name + " wgrad")
 
     def _simple(self, fn, dims, *args):
         dims_a = (ctypes.c_int * 4)(*dims)

@@ -1,7 +1,7 @@
 """Write the golden fixtures of tests/golden/ from the FP64 oracle.
 
     python tests/golden/make_golden.py small    # config-1 (32^3) fixture, seconds
-    python tests/golden/make_golden.py big      # config-2 (256^3) C_eff, ~1 h on 8 cores
+    python tests/golden/make_golden.py big      # config-2 (256^3) C_eff, ~2 h on 8 cores (resumable)
 
 The microstructures come from the seeded cross-ply generator (RSA placement,
 splitmix64), restated in oracle/microstructure_oracle.py; the fields are not
@@ -48,27 +48,38 @@ def small():
 
 
 def big():
+    """One load case at a time; finished rows are kept in a side file so an
+    interrupted run resumes where it stopped."""
     cfg = CONFIG2
     t0 = time.time()
     lam, mu, vf, nf = fields(cfg)
     gs = O.GridSpec(*cfg["shape"], *cfg["extents"])
-    workers = len(os.sched_getaffinity(0))
-    C = np.zeros((6, 6))
-    its = []
+    workers = int(os.environ.get("GOLDEN_WORKERS", len(os.sched_getaffinity(0))))
+    partial = os.path.join(HERE, "crossply256_ceff.partial.json")
+    done = {}
+    if os.path.exists(partial):
+        with open(partial) as f:
+            done = {int(k): v for k, v in json.load(f).items()}
     for k in range(6):
+        if k in done:
+            continue
         e = np.zeros(6)
         e[k] = 1.0
         _, sigma, it, hist = O.lippmann_schwinger_rfft(
             lam, mu, e, gs, rtol=1e-9, depth=5, maxiter=2000, workers=workers,
             callback=lambda i, r: print(f"  load case {k} it {i} R {r:.3e} t={time.time() - t0:.0f}s", flush=True)
             if i % 10 == 0 else None)
-        C[k] = sigma.mean(axis=(1, 2, 3))
-        its.append(it)
+        done[k] = dict(row=sigma.mean(axis=(1, 2, 3)).tolist(), iters=int(it))
+        with open(partial, "w") as f:
+            json.dump({str(i): v for i, v in done.items()}, f)
         print(f"load case {k}: {it} iterations, {time.time() - t0:.0f}s", flush=True)
+    C = np.array([done[k]["row"] for k in range(6)])
+    its = [done[k]["iters"] for k in range(6)]
     out = dict(config=cfg, achieved_vf=vf, n_fibres=nf, method="anderson m=5, float64, rtol 1e-9",
                ceff=C.tolist(), iters=its)
     with open(os.path.join(HERE, "crossply256_ceff.json"), "w") as f:
         json.dump(out, f, indent=1)
+    os.remove(partial)
     print("wrote crossply256_ceff.json", its, f"{time.time() - t0:.0f}s")
 
 

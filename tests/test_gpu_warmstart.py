@@ -136,3 +136,45 @@ def test_invalid_arguments_are_rejected(gpu):
     rc = lib.ls_ws_conv3d(dims, ctypes.c_void_p(16), 8, 6, ctypes.c_void_p(16), 192, None, 16, ctypes.c_void_p(16),
                           16, 0, None, 0, None, 0, None)
     assert rc == -1
+
+
+def test_trained_warm_start_vs_uniform_start(gpu):
+    """What the trained network (jaxmaterials_amd/warmstart/crossply64.npz)
+    does for a held-out 64^3 microstructure of its training family: its
+    prediction error, the initial residual of the start the solver keeps
+    (never above the uniform start's: the solver falls back), the iterations
+    with and without it, and that both starts reach the same C_eff."""
+    import torch
+
+    from jaxmaterials_amd.common import GridSpec
+    from jaxmaterials_amd.data import crossply_lame
+    from jaxmaterials_amd.solver.device import LSSolver, unit_load_cases
+    from jaxmaterials_amd.warmstart import WarmStartNet
+    from jaxmaterials_amd.warmstart.train import DEFAULT_WEIGHTS, warm_start_solve
+
+    net = WarmStartNet.load(DEFAULT_WEIGHTS, device=gpu)
+    n = 64
+    shape, ext = (n,) * 3, (float(n),) * 3
+    gs = GridSpec(*shape, *ext)
+    lam, mu, _, _ = crossply_lame(shape, ext, 4, 6.0, 0.5, seed=7)  # training seeds start at 1 000 000
+    loads = unit_load_cases()
+    for depth in (5, 0):
+        solver = LSSolver(gs, nrhs=6, depth=depth)
+        ref = solver.solve(lam, mu, loads, rtol=1e-6, maxiter=3000)
+        eps_ref = ref["epsilon"].clone()
+        pred = torch.empty_like(eps_ref)
+        net.forward(lam.reshape((1,) + shape), mu.reshape((1,) + shape), torch.eye(6, device=gpu), 6, pred)
+        eb = torch.eye(6, device=gpu).view(6, 6, 1, 1, 1)
+        err = float(torch.linalg.vector_norm(pred - eps_ref) / torch.linalg.vector_norm(eps_ref - eb))
+        r0 = warm_start_solve(net, solver, lam, mu, loads, rtol=1e-6, maxiter=0, check=False)["residual"]
+        out = warm_start_solve(net, solver, lam, mu, loads, rtol=1e-6, maxiter=3000)
+        C0, Cw = ref["sigma_mean"], out["sigma_mean"]
+        sig = np.abs(C0) > 1e-3 * np.abs(C0).max()
+        dC = float((np.abs(Cw - C0)[sig] / np.abs(C0)[sig]).max())
+        print(f"depth {depth}: prediction rel L2 err {err:.3f}; initial residual vs uniform "
+              f"{np.round(r0, 3).tolist()}; iterations uniform {ref['iters'].tolist()} warm {out['iters'].tolist()}; "
+              f"C_eff max rel diff {dC:.2e}")
+        assert err < 0.25
+        assert np.all(r0 <= 1.0 + 1e-12)
+        assert out["converged"] and dC < 1e-5
+        solver.close()
