@@ -67,15 +67,28 @@ ls_solver *ls_solver_create(const int *voxels, const float *extents, int nrhs,
  * uniform start R(eps_bar) in every case; rel_ref > 0 overrides it.
  * flags: LS_ABSOLUTE_RESIDUAL = residual ||xi . sigma_hat|| without the
  * division by ||sigma_hat(0)|| (for polarisation-driven solves such as the
- * adjoint, whose stress may have zero mean).  Returns 0 if all converged,
- * 1 if some did not, -1 on error. */
+ * adjoint, whose stress may have zero mean); LS_GRAPH = run the iterations
+ * between two host synchronisations as one replay of a captured HIP graph
+ * (captured on first use, again when the fields or tolerances change; the
+ * results are identical to the launch-by-launch path).  Returns 0 if all
+ * converged, 1 if some did not, -1 on error. */
 #define LS_ABSOLUTE_RESIDUAL 1
+#define LS_GRAPH 2
 int ls_solver_solve(ls_solver *s, const float *d_lambda, const float *d_mu,
                     const int *rhs_mat, const float *eps_bar,
                     const float *d_tau, float *d_eps, float *d_sigma,
                     int warm_start, float rtol, float atol, int maxiter,
                     double rel_ref, int flags, int verbose, int *iters,
                     double *residual, double *sigma_mean);
+
+/* Autotuner: time every candidate tile (lines per block, register elements
+ * and threads) of the z-line, y and x passes on the solver's own buffers and
+ * keep the fastest for later solves.  d_lambda, d_mu [nmat][N] and d_eps
+ * [nrhs][6][N] are device buffers (d_eps is overwritten); reps launches per
+ * candidate.  report (may be NULL) receives the chosen tiles and times.
+ * Returns 0, or -1 on error. */
+int ls_solver_autotune(ls_solver *s, const float *d_lambda, const float *d_mu,
+                       float *d_eps, int reps, char *report, int report_cap);
 
 /* Kernel time of the last solve (recorded with HIP events on the solver's
  * stream when the environment variable LS_TIMING is set), ms and launch

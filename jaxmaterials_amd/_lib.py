@@ -31,6 +31,10 @@ SIGNATURES = {
         [ctypes.c_void_p] * 8 + [ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_double,
                                  ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     ),
+    "ls_solver_autotune": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int],
+    ),
     "ls_solver_last_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
     "ls_solver_destroy": (None, [ctypes.c_void_p]),
     "ls_crossply_lame": (

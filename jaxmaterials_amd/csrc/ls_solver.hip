@@ -108,6 +108,7 @@ AxisPlan to_dev(const HostAxis& a) {
 }
 
 constexpr size_t kMaxLds = 60 * 1024;
+constexpr int kMaxTm = 256;  // largest x-pass tile (modes per block); bounds the d_base padding
 
 struct Tile {
   int lines = 0;  // Lz (z), T (y) or Tm (x)
@@ -172,6 +173,11 @@ struct ls_solver {
   double* aa_gram = nullptr;
   int* aa_head = nullptr;
   int* aa_filled = nullptr;
+  // captured chunk of iterations between two host synchronisations (LS_GRAPH)
+  hipGraphExec_t gexec = nullptr;
+  const void* gkey[3] = {nullptr, nullptr, nullptr};
+  double gtol[2] = {0.0, 0.0};
+  int gsteps = 0, gabs = -1;
   // timing of the last solve (LS_TIMING set): HIP events on the solver stream
   static constexpr int kKinds = 6;  // z (plain), y, x, other, z Anderson residual, z Anderson mix
   double t_ms[kKinds] = {0, 0, 0, 0, 0, 0};
@@ -383,6 +389,7 @@ std::vector<int> active_mask(const ls_solver* s) {
 }
 
 void free_solver(ls_solver* s) {
+  if (s && s->gexec) (void)hipGraphExecDestroy(s->gexec);
   if (!s) return;
   for (auto& a : s->ax) {
     (void)hipFree(a.d_tw);
@@ -400,6 +407,18 @@ void free_solver(ls_solver* s) {
   for (auto e : s->tev) (void)hipEventDestroy(e);
   if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
+}
+
+// Block counts and the d_base layout of the current FP32 tiles.  The d_base
+// stride per load case is fixed for any x tile of at most kMaxTm modes.
+void apply_tiles(ls_solver* s) {
+  s->nblk_z = (int)((s->nx * (long long)s->ny + s->tz.lines - 1) / s->tz.lines);
+  s->ntiles_y = (6 * s->P + s->ty.lines - 1) / s->ty.lines;
+  s->nblk_y = s->nx * s->ntiles_y;
+  s->nblk_x = (int)(((long long)s->ny * s->P + s->tx.lines - 1) / s->tx.lines);
+  s->dl.f_Tm = make_fastdiv(s->tx.lines);
+  s->dl.nx = s->nx;
+  s->dl.per_rhs = ((long long)s->ny * s->P + kMaxTm) * s->nx * 3;
 }
 
 int create_impl(ls_solver* s, const int* voxels, const float* extents, int nrhs, int nmat, int depth,
@@ -456,10 +475,11 @@ int create_impl(ls_solver* s, const int* voxels, const float* extents, int nrhs,
     fprintf(stderr, "liblippmannschwinger: no FFT tile fits for grid %d x %d x %d\n", s->nx, s->ny, s->nz);
     return -1;
   }
-  s->nblk_z = (int)((s->nx * (long long)s->ny + s->tz.lines - 1) / s->tz.lines);
-  s->ntiles_y = (6 * s->P + s->ty.lines - 1) / s->ty.lines;
-  s->nblk_y = s->nx * s->ntiles_y;
-  s->nblk_x = (int)(((long long)s->ny * s->P + s->tx.lines - 1) / s->tx.lines);
+  if (s->tx.lines > kMaxTm) {
+    fprintf(stderr, "liblippmannschwinger: x tile of %d modes exceeds %d\n", s->tx.lines, kMaxTm);
+    return -1;
+  }
+  apply_tiles(s);
   s->nblk_z64 = (int)((s->nx * (long long)s->ny + s->tz64.lines - 1) / s->tz64.lines);
   s->ntiles_y64 = (6 * s->P + s->ty64.lines - 1) / s->ty64.lines;
   s->nblk_y64 = s->nx * s->ntiles_y64;
@@ -486,12 +506,11 @@ int create_impl(ls_solver* s, const int* voxels, const float* extents, int nrhs,
   g.f_nz = make_fastdiv(s->nz);
   g.f_P = make_fastdiv(s->P);
   g.lsz = lsz;
-  s->dl.f_Tm = make_fastdiv(s->tx.lines);
-  s->dl.nx = s->nx;
-  s->dl.per_rhs = (long long)s->nblk_x * s->nx * s->tx.lines * 3;
+  // buffers whose size depends on the FP32 tiles are sized for any tile (so the
+  // tiles can change between solves, ls_solver_autotune)
   if (dalloc(&s->spec, (size_t)nrhs * s->M * 6) || dalloc(&s->spec64, (size_t)s->M * 6) ||
       dalloc(&s->base, (size_t)nrhs * 6 * s->N) || dalloc(&s->dbase, (size_t)nrhs * s->dl.per_rhs) ||
-      dalloc(&s->partials, (size_t)nrhs * s->nblk_x) || dalloc(&s->dc, (size_t)nrhs * 6) ||
+      dalloc(&s->partials, (size_t)nrhs * s->ny * s->P) || dalloc(&s->dc, (size_t)nrhs * 6) ||
       dalloc(&s->d_rhs, nrhs) || dalloc(&s->d_st, nrhs) || dalloc(&s->d_sel, nrhs))
     return -1;
   LS_HIP(hipHostMalloc((void**)&s->h_st, nrhs * sizeof(RhsState), hipHostMallocDefault));
@@ -506,7 +525,7 @@ int create_impl(ls_solver* s, const int* voxels, const float* extents, int nrhs,
   if (depth > 0) {
     const size_t hist = (size_t)nrhs * s->m1 * 6 * s->N;
     if (dalloc(&s->aaG, hist) || dalloc(&s->aaR, hist) || dalloc(&s->aa_alpha, (size_t)nrhs * s->m1) ||
-        dalloc(&s->aa_dots, (size_t)nrhs * s->nblk_z * s->m1) || dalloc(&s->aa_gram, (size_t)nrhs * s->m1 * s->m1) ||
+        dalloc(&s->aa_dots, (size_t)nrhs * s->nx * s->ny * s->m1) || dalloc(&s->aa_gram, (size_t)nrhs * s->m1 * s->m1) ||
         dalloc(&s->aa_head, nrhs) || dalloc(&s->aa_filled, nrhs))
       return -1;
   }
@@ -554,6 +573,38 @@ void iterate(ls_solver* s, const float* lam, const float* mu, float* eps, double
     zline<Z_AAMIX>(s, lam, mu, eps);
   }
   if (evaluate_after) evaluate(s, rtol, atol, 0);
+}
+
+// `steps` iterations (each followed by its residual evaluation) as one
+// replay of a captured HIP graph.  The graph bakes in the kernel arguments, so
+// it is captured again when the fields, tolerances or chunk length change.
+// The captured work has no memset nodes: iterate() launches kernels only.
+int iterate_graph(ls_solver* s, const float* lam, const float* mu, float* eps, double rtol, double atol,
+                  int steps) {
+  const bool same = s->gexec && s->gkey[0] == lam && s->gkey[1] == mu && s->gkey[2] == eps && s->gtol[0] == rtol &&
+                    s->gtol[1] == atol && s->gsteps == steps && s->gabs == s->absolute;
+  if (!same) {
+    if (s->gexec) {
+      LS_HIP(hipGraphExecDestroy(s->gexec));
+      s->gexec = nullptr;
+    }
+    hipGraph_t graph = nullptr;
+    LS_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < steps; ++k) iterate(s, lam, mu, eps, rtol, atol, true);
+    LS_HIP(hipStreamEndCapture(s->stream, &graph));
+    const hipError_t e = hipGraphInstantiate(&s->gexec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    LS_HIP(e);
+    s->gkey[0] = lam;
+    s->gkey[1] = mu;
+    s->gkey[2] = eps;
+    s->gtol[0] = rtol;
+    s->gtol[1] = atol;
+    s->gsteps = steps;
+    s->gabs = s->absolute;
+  }
+  LS_HIP(hipGraphLaunch(s->gexec, s->stream));
+  return 0;
 }
 
 // Move the corrections of all active load cases into their bases and
@@ -721,6 +772,8 @@ int solve_impl(ls_solver* s, const float* d_lambda, const float* d_mu, const int
     printf("     %4d          %8.4e  %8.4e\n", 0, s->h_st[0].R, s->h_st[0].R / R0_print);
   }
   const int sync_every = verbose == 2 ? 1 : env_int("LS_SYNC_EVERY", s->N >= (1LL << 21) ? 2 : 8);
+  // graph replay of the iteration chunks (not with per-kernel timing, whose events it would capture)
+  const bool use_graph = (flags & 2) && !s->timing && sync_every > 1;
   bool q7_done = false;
   std::vector<int> q7(s->nrhs, 0);
   for (int it = 1; it <= maxiter; ++it) {
@@ -751,6 +804,12 @@ int solve_impl(ls_solver* s, const float* d_lambda, const float* d_mu, const int
       k_stress<<<grid, kThreads, 0, s->stream>>>(s->gd, s->d_rhs, s->d_sel, d_lambda, d_mu, s->base, d_eps, 0,
                                                   d_tau, d_sigma, s->part6, s->chunk_stress);
       q7_done = true;
+    }
+    // a whole chunk up to the next synchronisation as one graph replay
+    if (use_graph && (it - 1) % sync_every == 0 && it + sync_every - 1 < maxiter) {
+      if (iterate_graph(s, d_lambda, d_mu, d_eps, drtol, datol, sync_every)) return -1;
+      it += sync_every - 1;
+      continue;
     }
     iterate(s, d_lambda, d_mu, d_eps, drtol, datol, !last);
   }
@@ -812,6 +871,107 @@ int solve_impl(ls_solver* s, const float* d_lambda, const float* d_mu, const int
   return rc;
 }
 
+// ---- autotuner -------------------------------------------------------------
+// Candidate tiles of one axis: for each wanted unit count and each register
+// budget, the largest fitting tile (duplicates dropped).
+std::vector<Tile> tile_candidates(const HostAxis& a, const std::vector<int>& wants, int lines_per_unit,
+                                  size_t bytes_per_unit, int max_units) {
+  std::vector<Tile> out;
+  for (int w : wants) {
+    for (const Budget& b : kBudget32) {
+      Tile t;
+      if (!choose_tile(a, std::max(1, std::min(w, max_units)), lines_per_unit, bytes_per_unit, {b}, t)) continue;
+      bool dup = false;
+      for (const Tile& o : out) dup = dup || (o.lines == t.lines && o.E == t.E && o.NT == t.NT);
+      if (!dup) out.push_back(t);
+    }
+  }
+  return out;
+}
+
+template <typename F>
+float time_launches(ls_solver* s, int reps, F&& launch) {
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess) return -1.f;
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return -1.f;
+  }
+  launch();  // warm-up
+  (void)hipEventRecord(e0, s->stream);
+  for (int i = 0; i < reps; ++i) launch();
+  (void)hipEventRecord(e1, s->stream);
+  float ms = -1.f;
+  if (hipEventSynchronize(e1) == hipSuccess) (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return hipGetLastError() == hipSuccess ? ms / reps : -1.f;
+}
+
+// Time each candidate tile of the three FP32 passes on the solver's own
+// buffers (zeroed spectrum, strain and d_base; every load case active) and keep
+// the fastest.  The z tile is timed with the plain z-line pass; the Anderson
+// z-line passes share its LDS and register layout.
+int autotune_impl(ls_solver* s, const float* lam, const float* mu, float* eps, int reps, std::string& rep) {
+  const bool timing = s->timing;
+  s->timing = false;
+  std::vector<RhsDesc> desc(s->nrhs);
+  for (auto& d : desc) {
+    d.mat = 0;
+    for (float& e : d.eps_bar) e = 0.f;
+    d.lam0 = 1.f;
+    d.mu0 = 1.f;
+    d.cA = (float)(-1.0 / (double)s->N);
+    d.cB = (float)(2.0 / (3.0 * (double)s->N));
+  }
+  LS_HIP(hipMemcpyAsync(s->d_rhs, desc.data(), desc.size() * sizeof(RhsDesc), hipMemcpyHostToDevice, s->stream));
+  for (int r = 0; r < s->nrhs; ++r) {
+    s->h_st[r] = RhsState{};
+    s->h_st[r].active = 1;
+  }
+  if (put_state(s)) return -1;
+  LS_HIP(hipMemsetAsync(s->spec, 0, (size_t)s->nrhs * s->M * 6 * sizeof(float2), s->stream));
+  LS_HIP(hipMemsetAsync(eps, 0, (size_t)s->nrhs * 6 * s->N * sizeof(float), s->stream));
+  LS_HIP(hipMemsetAsync(s->dbase, 0, (size_t)s->nrhs * s->dl.per_rhs * sizeof(float2), s->stream));
+  LS_HIP(hipMemsetAsync(s->dc, 0, (size_t)s->nrhs * 6 * sizeof(float), s->stream));
+  const int lsz = s->gd.lsz;
+  auto sweep = [&](const char* name, Tile& cur, const std::vector<Tile>& cands, auto&& launch) -> int {
+    Tile best = cur;
+    float best_ms = -1.f;
+    char buf[160];
+    for (const Tile& t : cands) {
+      cur = t;
+      apply_tiles(s);
+      const float ms = time_launches(s, reps, launch);
+      if (ms < 0.f) return -1;
+      if (best_ms < 0.f || ms < best_ms) {
+        best_ms = ms;
+        best = t;
+      }
+    }
+    cur = best;
+    apply_tiles(s);
+    snprintf(buf, sizeof(buf), "%s: %d lines E %d NT %d %.3f ms (%zu candidates); ", name, best.lines, best.E,
+             best.NT, best_ms, cands.size());
+    rep += buf;
+    return 0;
+  };
+  const int wz = std::max(1, 3072 / (3 * s->nz));
+  const int wy = std::max(1, std::min(6 * s->P, 4096 / s->ny));
+  const int wx = std::max(1, 3072 / (6 * s->nx));
+  auto wants = [](int w) { return std::vector<int>{std::max(1, w / 4), std::max(1, w / 2), w, 2 * w, 4 * w}; };
+  if (sweep("z", s->tz, tile_candidates(s->ax[2], wants(wz), 3, (size_t)3 * lsz * sizeof(float2), 1 << 20),
+            [&] { zline<Z_UPD>(s, lam, mu, eps); }) ||
+      sweep("y", s->ty, tile_candidates(s->ax[1], wants(wy), 1, (size_t)s->ny * sizeof(float2), 6 * s->P),
+            [&] { ypass(s, true); }) ||
+      sweep("x", s->tx, tile_candidates(s->ax[0], wants(wx), 6, (size_t)6 * s->nx * sizeof(float2), kMaxTm),
+            [&] { xpass(s, 1); }))
+    return -1;
+  s->timing = timing;
+  LS_HIP(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
 }  // namespace
 
 // ============================================================================
@@ -837,6 +997,15 @@ int ls_solver_solve(ls_solver* s, const float* d_lambda, const float* d_mu, cons
   if (!s || !d_lambda || !d_mu || !eps_bar || !d_eps) return -1;
   return solve_impl(s, d_lambda, d_mu, rhs_mat, eps_bar, d_tau, d_eps, d_sigma, warm_start, rtol, atol, maxiter,
                     rel_ref, verbose, iters, residual, sigma_mean, false, flags);
+}
+
+int ls_solver_autotune(ls_solver* s, const float* d_lambda, const float* d_mu, float* d_eps, int reps, char* report,
+                       int report_cap) {
+  if (!s || !d_lambda || !d_mu || !d_eps || reps < 1) return -1;
+  std::string rep;
+  const int rc = autotune_impl(s, d_lambda, d_mu, d_eps, reps, rep);
+  if (report && report_cap > 0) snprintf(report, (size_t)report_cap, "%s", rep.c_str());
+  return rc;
 }
 
 int ls_solver_last_timing(ls_solver* s, double* ms, int* count, int nkinds) {

@@ -71,7 +71,7 @@ class LSSolver:
 
     def solve(self, lmbda, mu, epsilon_bar, rhs_mat=None, tau=None, epsilon=None, warm_start=False,
               want_sigma=False, rtol=1e-6, atol=0.0, maxiter=1000, rel_ref=0.0, absolute_residual=False,
-              verbose=0, check=True):
+              verbose=0, check=True, graph=False):
         """Solve all load cases.
 
         :arg lmbda, mu: GPU float32 [nmat, nx, ny, nz] (or [nx, ny, nz] when nmat == 1)
@@ -81,6 +81,7 @@ class LSSolver:
         :arg epsilon: optional GPU float32 [nrhs, 6, nx, ny, nz]; start field if warm_start
         :arg rel_ref: normaliser of rtol; 0 means the residual of the uniform start
         :arg absolute_residual: residual without the mean-stress normalisation
+        :arg graph: replay the iterations between host synchronisations as a captured HIP graph
         Returns a dict with epsilon, sigma (or None), iters, residual, sigma_mean (float64 [nrhs, 6]).
         """
         R = self.nrhs
@@ -102,11 +103,12 @@ class LSSolver:
         iters = np.zeros(R, dtype=np.int32)
         res = np.zeros(R, dtype=np.float64)
         smean = np.zeros((R, 6), dtype=np.float64)
-        torch.cuda.synchronize(lm.device)
+        torch.cuda.current_stream(lm.device).synchronize()
         rc = self.lib.ls_solver_solve(
             self._h, _ptr(lm), _ptr(m), mats.ctypes.data_as(ctypes.c_void_p), eb.ctypes.data_as(ctypes.c_void_p),
             _ptr(tau), _ptr(epsilon), _ptr(sigma), int(bool(warm_start)), float(rtol), float(atol), int(maxiter),
-            float(rel_ref), 1 if absolute_residual else 0, int(verbose), iters.ctypes.data_as(ctypes.c_void_p),
+            float(rel_ref), (1 if absolute_residual else 0) | (2 if graph else 0), int(verbose),
+            iters.ctypes.data_as(ctypes.c_void_p),
             res.ctypes.data_as(ctypes.c_void_p), smean.ctypes.data_as(ctypes.c_void_p),
         )
         if rc < 0:
@@ -115,6 +117,25 @@ class LSSolver:
             raise RuntimeError(f"Solver failed to converge after {maxiter} iterations")
         return {"epsilon": epsilon, "sigma": sigma, "iters": iters, "residual": res, "sigma_mean": smean,
                 "converged": rc == 0}
+
+    def autotune(self, lmbda, mu, epsilon=None, reps=3):
+        """Time the candidate FFT tiles of the three passes on this solver's
+        buffers and keep the fastest (``ls_solver_autotune``).  epsilon, if
+        given, is used (and overwritten) as the strain scratch.  Returns the
+        tuner's report."""
+        lm = lmbda.reshape((self.nmat,) + self.shape) if lmbda.dim() == 3 else lmbda
+        m = mu.reshape((self.nmat,) + self.shape) if mu.dim() == 3 else mu
+        _check_dev("lmbda", lm, (self.nmat,) + self.shape)
+        _check_dev("mu", m, (self.nmat,) + self.shape)
+        if epsilon is None:
+            epsilon = torch.empty((self.nrhs, 6) + self.shape, dtype=torch.float32, device=lm.device)
+        _check_dev("epsilon", epsilon, (self.nrhs, 6) + self.shape)
+        buf = ctypes.create_string_buffer(1024)
+        torch.cuda.current_stream(lm.device).synchronize()
+        rc = self.lib.ls_solver_autotune(self._h, _ptr(lm), _ptr(m), _ptr(epsilon), int(reps), buf, len(buf))
+        if rc != 0:
+            raise RuntimeError("ls_solver_autotune failed (see stderr)")
+        return buf.value.decode()
 
     KINDS = ("z", "y", "x", "other", "z_aa_residual", "z_aa_mix")
 
