@@ -21,6 +21,7 @@
 
 #include "../../include/lippmann_schwinger.h"
 #include "ls_kernels.hpp"
+#include "ls_passes_reg.hpp"
 
 using namespace lsk;
 
@@ -112,10 +113,48 @@ constexpr int kMaxTm = 256;  // largest x-pass tile (modes per block); bounds th
 
 struct Tile {
   int lines = 0;  // Lz (z), T (y) or Tm (x)
-  int E = 0;      // register elements per thread
+  int E = 0;      // register elements per thread (LDS engine)
   int NT = 0;     // threads per block
   size_t lds = 0;
+  int reg = 0;    // 1: register two-stage kernel (k_ypass_r / k_xpass_r)
 };
+
+// Register two-stage kernels compiled for the axis lengths of the workloads
+// (n = A B; lines = T columns for y, TM modes for x).  The first entry of a
+// length is its default; the autotuner times all of them against the LDS engine.
+#define LS_REG_Y_LIST(F)                                                                         \
+  F(256, 16, 16, 16, 256) F(256, 16, 16, 8, 128) F(128, 16, 8, 32, 256) F(128, 16, 8, 48, 384)  \
+  F(96, 12, 8, 32, 256) F(96, 12, 8, 64, 512) F(64, 8, 8, 32, 256) F(64, 8, 8, 64, 512)         \
+  F(48, 6, 8, 32, 256) F(48, 6, 8, 64, 512) F(32, 8, 4, 64, 256) F(32, 8, 4, 128, 512)
+#define LS_REG_X_LIST(F)                                                                        \
+  F(256, 32, 8, 4, 192) F(256, 32, 8, 2, 128) F(128, 16, 8, 4, 192) F(128, 16, 8, 8, 384)      \
+  F(96, 12, 8, 4, 192) F(96, 12, 8, 8, 384) F(64, 8, 8, 4, 192) F(64, 8, 8, 8, 384)            \
+  F(48, 6, 8, 4, 192) F(48, 6, 8, 8, 384) F(32, 8, 4, 8, 192) F(32, 8, 4, 16, 384)
+
+struct RegCfg {
+  int n, A, B, lines, NT;
+};
+#define LS_AS_CFG(NV, AV, BV, LV, NTV) {NV, AV, BV, LV, NTV},
+const RegCfg kRegY[] = {LS_REG_Y_LIST(LS_AS_CFG)};
+const RegCfg kRegX[] = {LS_REG_X_LIST(LS_AS_CFG)};
+#undef LS_AS_CFG
+
+// Register-kernel tiles of an axis; lines_per_unit = 1 (y columns) or 6 (x modes).
+std::vector<Tile> reg_tiles(const RegCfg* list, size_t count, int n, int lines_per_unit) {
+  std::vector<Tile> out;
+  for (size_t i = 0; i < count; ++i) {
+    const RegCfg& c = list[i];
+    if (c.n != n) continue;
+    Tile t;
+    t.lines = c.lines;
+    t.NT = c.NT;
+    t.reg = 1;
+    const int np = c.n + c.n / c.B + 1;
+    t.lds = (size_t)c.lines * lines_per_unit * np * sizeof(float2);
+    out.push_back(t);
+  }
+  return out;
+}
 
 struct Budget {
   int E, NT;
@@ -293,6 +332,22 @@ void zline(ls_solver* s, const float* lam, const float* mu, float* eps) {
 void ypass(ls_solver* s, bool inv) {
   const unsigned grid = (unsigned)s->nblk_y * s->nrhs;
   const size_t lds = s->ty.lds;
+  if (s->ty.reg) {
+#define LS_YR(NV, AV, BV, LV, NTV)                                                                      \
+  else if (s->ny == NV && s->ty.lines == LV && s->ty.NT == NTV) {                                       \
+    if (inv)                                                                                            \
+      k_ypass_r<AV, BV, LV, NTV, true><<<grid, NTV, lds, s->stream>>>(s->gd, s->d_st, s->spec, s->ntiles_y);  \
+    else                                                                                                \
+      k_ypass_r<AV, BV, LV, NTV, false><<<grid, NTV, lds, s->stream>>>(s->gd, s->d_st, s->spec, s->ntiles_y); \
+  }
+    if (false) {
+    }
+    LS_REG_Y_LIST(LS_YR)
+    else fprintf(stderr, "liblippmannschwinger: no register y kernel for ny=%d T=%d\n", s->ny, s->ty.lines);
+#undef LS_YR
+    mark(s, 1);
+    return;
+  }
   const FastDiv fT = make_fastdiv(s->ty.lines);
 #define LS_Y(EV, NTV)                                                                                               \
   if (inv)                                                                                                          \
@@ -308,6 +363,20 @@ void xpass(ls_solver* s, int do_gamma, bool with_base = true) {
   const unsigned grid = (unsigned)s->nblk_x * s->nrhs;
   const size_t lds = s->tx.lds;
   const float2* db = with_base ? s->dbase : nullptr;
+  if (s->tx.reg) {
+#define LS_XR(NV, AV, BV, LV, NTV)                                                                        \
+  else if (s->nx == NV && s->tx.lines == LV && s->tx.NT == NTV) {                                         \
+    k_xpass_r<AV, BV, LV, NTV><<<grid, NTV, lds, s->stream>>>(s->gd, s->d_rhs, s->d_st, s->spec, db, s->partials, \
+                                                              s->dc, s->nblk_x, s->dl.per_rhs, do_gamma);   \
+  }
+    if (false) {
+    }
+    LS_REG_X_LIST(LS_XR)
+    else fprintf(stderr, "liblippmannschwinger: no register x kernel for nx=%d TM=%d\n", s->nx, s->tx.lines);
+#undef LS_XR
+    mark(s, 2);
+    return;
+  }
   const FastDiv fTm = make_fastdiv(s->tx.lines), fcols = make_fastdiv(6 * s->tx.lines);
 #define LS_X(EV, NTV)                                                                                      \
   k_xpass<EV, NTV><<<grid, NTV, lds, s->stream>>>(s->gd, s->d_rhs, s->d_st, s->spec, db, s->partials, s->dc, \
@@ -474,6 +543,12 @@ int create_impl(ls_solver* s, const int* voxels, const float* extents, int nrhs,
   if (!ok32 || !ok64) {
     fprintf(stderr, "liblippmannschwinger: no FFT tile fits for grid %d x %d x %d\n", s->nx, s->ny, s->nz);
     return -1;
+  }
+  if (env_int("LS_REG_FFT", 1)) {
+    const std::vector<Tile> ry = reg_tiles(kRegY, sizeof(kRegY) / sizeof(kRegY[0]), s->ny, 1);
+    const std::vector<Tile> rx = reg_tiles(kRegX, sizeof(kRegX) / sizeof(kRegX[0]), s->nx, 6);
+    if (!ry.empty()) s->ty = ry[0];
+    if (!rx.empty()) s->tx = rx[0];
   }
   if (s->tx.lines > kMaxTm) {
     fprintf(stderr, "liblippmannschwinger: x tile of %d modes exceeds %d\n", s->tx.lines, kMaxTm);
@@ -951,8 +1026,8 @@ int autotune_impl(ls_solver* s, const float* lam, const float* mu, float* eps, i
     }
     cur = best;
     apply_tiles(s);
-    snprintf(buf, sizeof(buf), "%s: %d lines E %d NT %d %.3f ms (%zu candidates); ", name, best.lines, best.E,
-             best.NT, best_ms, cands.size());
+    snprintf(buf, sizeof(buf), "%s: %s %d lines E %d NT %d %.3f ms (%zu candidates); ", name,
+             best.reg ? "register" : "lds", best.lines, best.E, best.NT, best_ms, cands.size());
     rep += buf;
     return 0;
   };
@@ -960,12 +1035,13 @@ int autotune_impl(ls_solver* s, const float* lam, const float* mu, float* eps, i
   const int wy = std::max(1, std::min(6 * s->P, 4096 / s->ny));
   const int wx = std::max(1, 3072 / (6 * s->nx));
   auto wants = [](int w) { return std::vector<int>{std::max(1, w / 4), std::max(1, w / 2), w, 2 * w, 4 * w}; };
+  std::vector<Tile> cy = tile_candidates(s->ax[1], wants(wy), 1, (size_t)s->ny * sizeof(float2), 6 * s->P);
+  std::vector<Tile> cx = tile_candidates(s->ax[0], wants(wx), 6, (size_t)6 * s->nx * sizeof(float2), kMaxTm);
+  for (const Tile& t : reg_tiles(kRegY, sizeof(kRegY) / sizeof(kRegY[0]), s->ny, 1)) cy.push_back(t);
+  for (const Tile& t : reg_tiles(kRegX, sizeof(kRegX) / sizeof(kRegX[0]), s->nx, 6)) cx.push_back(t);
   if (sweep("z", s->tz, tile_candidates(s->ax[2], wants(wz), 3, (size_t)3 * lsz * sizeof(float2), 1 << 20),
             [&] { zline<Z_UPD>(s, lam, mu, eps); }) ||
-      sweep("y", s->ty, tile_candidates(s->ax[1], wants(wy), 1, (size_t)s->ny * sizeof(float2), 6 * s->P),
-            [&] { ypass(s, true); }) ||
-      sweep("x", s->tx, tile_candidates(s->ax[0], wants(wx), 6, (size_t)6 * s->nx * sizeof(float2), kMaxTm),
-            [&] { xpass(s, 1); }))
+      sweep("y", s->ty, cy, [&] { ypass(s, true); }) || sweep("x", s->tx, cx, [&] { xpass(s, 1); }))
     return -1;
   s->timing = timing;
   LS_HIP(hipStreamSynchronize(s->stream));

@@ -217,3 +217,28 @@ def test_fp32_noise_floor_crossply(gpu, n):
     print(f"crossply {n}^3: iter gpu={out['iters'][0]} oracle={it_ref} strain relL2={err:.2e}", flush=True)
     assert abs(int(out["iters"][0]) - it_ref) <= 1
     assert err <= 1e-5
+
+
+# Axis lengths with a register two-stage FFT kernel in the y and x passes
+# (256 = 16x16 / 32x8, 128, 96, 64, 48, 32); each grid puts one of them on x
+# and another on y with a short z axis so the FP64 oracle stays cheap.
+LONG_AXES = [(256, 96, 8), (96, 256, 8), (128, 48, 10), (48, 128, 10), (32, 64, 12), (64, 32, 12)]
+
+
+@pytest.mark.parametrize("shape", LONG_AXES, ids=lambda s: "x".join(map(str, s)))
+def test_register_fft_lengths_match_oracle(gpu, shape):
+    """Strain after a fixed number of plain and Anderson updates against the FP64
+    oracle, on grids whose x and y lengths select the register FFT kernels."""
+    gs = GridSpec(*shape, 1.0, 0.9, 0.3)
+    lmbda, mu = random_material(gs, seed=17, lo=0.5, hi=1.5)
+    lam_t = torch.from_numpy(lmbda).to(gpu)
+    mu_t = torch.from_numpy(mu).to(gpu)
+    for depth in (0, 3):
+        s = LSSolver(gs, nrhs=1, depth=depth)
+        eb = EPS_BAR[None].astype(np.float32)
+        out = s.solve(lam_t, mu_t, eb, rtol=1e-20, atol=0.0, maxiter=6, check=False)
+        e_k, _, _ = oracle_fixed_updates(lmbda, mu, EPS_BAR, gs, 6, depth=depth)
+        err = rel_l2(out["epsilon"][0].cpu().numpy(), e_k)
+        print(f"register FFT {shape} depth {depth}: strain relL2 after 6 updates={err:.2e}", flush=True)
+        assert err <= 1e-5
+        s.close()
