@@ -50,6 +50,7 @@ struct GridDev {
   const float* cy;
   const float* sz;
   const float* cz;
+  const float2* phx;  // e^{+i pi k / nx}, k < nx (x pass with three-line transforms)
   AxisPlan px, py, pz;
   FastDiv f_nz, f_P;  // divisions by nz and P
   int lsz;            // padded LDS line stride of the z-line kernels

@@ -230,3 +230,235 @@ __global__ void __launch_bounds__(NT) k_xpass_r(GridDev g, const RhsDesc* __rest
 }
 
 }  // namespace lsk
+
+namespace lsk {
+
+// ---------------------------------------------------------------------------
+// x pass transforming three lines each way instead of six.
+//
+// With xi~ = (alpha s_x(k), beta c_x(k), gamma c_x(k)), s_x = sin(pi k / nx),
+// c_x = cos(pi k / nx) (alpha = gx c_y c_z, beta = gy s_y c_z, gamma = gz c_y s_z
+// depend on the mode (ky, kz) only), the divergence of a line is
+//   d_i(k) = alpha s_x F[sigma_ix] + c_x F[w_i],  w_i = beta sigma_iy + gamma sigma_iz,
+// and with s_x = e^{-i pi k/n} (e^{2 pi i k/n} - 1) / 2i,
+//          c_x = e^{-i pi k/n} (e^{2 pi i k/n} + 1) / 2:
+//   d_i(k) = e^{-i pi k/n} F[h_i](k),
+//   h_i(x) = alpha (sigma_ix(x+1) - sigma_ix(x)) / 2i + (w_i(x) + w_i(x+1)) / 2,
+// so the forward FFT along x acts on the three lines h_i.  Gamma0 in terms
+// of the divergence is r_hat = sym(xi~ (x) u), u = (cA v + cB rho n) / |xi~|
+// (v = d / |xi~|, n = xi~ / |xi~|, rho = n . v): with u = e^{-i pi k/n} u',
+//   r_ij(x) = (P_ij + P_ji) / 2,  P_xj = alpha (g_j(x) - g_j(x-1)) / 2i,
+//   P_yj = beta (g_j(x) + g_j(x-1)) / 2,  P_zj = gamma (g_j(x) + g_j(x-1)) / 2,
+// g = IFFT[u'], so the inverse FFT along x also acts on three lines.  The
+// residual |d|^2 = |F[h]|^2 is unchanged; d_base (the FP64 divergence of the
+// base stress) enters as e^{+i pi k/n} d_base.  Same results as k_xpass up to
+// FP32 rounding (lippmann_schwinger.py:118-143, fourier.py:115-186).
+//
+// LDS: h and then g lines [x][3 TM] in the first 3 TM N slots, the stage-1
+// transforms in RegLayout after them; per-mode coefficients in a small table.
+template <int A, int B, int TM, int NT>
+__global__ void __launch_bounds__(NT) k_xpass_h(GridDev g, const RhsDesc* __restrict__ rhs,
+                                                const RhsState* __restrict__ st, float2* __restrict__ spec,
+                                                const float2* __restrict__ dbase, double* __restrict__ partials,
+                                                float* __restrict__ dc, int ntiles, long long db_per_rhs,
+                                                int do_gamma) {
+  constexpr int N = A * B;
+  constexpr int LH = 3 * TM;  // h / g lines
+  using L = RegLayout<N, B>;
+  extern __shared__ float2 lds[];
+  float2* H = lds;           // [x][LH]
+  float2* W = lds + LH * N;  // RegLayout lines 3m + i
+  __shared__ double red[NT / 64];
+  // per mode: alpha, beta, gamma (xi~ = (alpha s_x, beta c_x, gamma c_x)),
+  // c_y c_z, s_y c_z, c_y s_z and the Hermitian weight
+  __shared__ float coef[TM][4];
+  __shared__ int valid[TM];
+  const int lin = xcd_remap(blockIdx.x, gridDim.x);
+  const int r = lin / ntiles;
+  const int tile = lin - r * ntiles;
+  if (!st[r].active) return;
+  const int P = g.P;
+  const int nm = g.ny * P;
+  const int m0 = tile * TM;
+  const long long colmax = 6LL * nm;
+  float2* S = spec + (long long)r * g.M * 6 + 6LL * m0;
+  const float2* DB = dbase ? dbase + (long long)r * db_per_rhs + (long long)tile * N * TM * 3 : nullptr;
+  const float2* __restrict__ tw = g.px.tw;
+  const RhsDesc d = rhs[r];
+  if (threadIdx.x < TM) {
+    const int gm = m0 + threadIdx.x;
+    const int ok = gm < nm;
+    const int ky = ok ? fdiv(gm, g.f_P) : 0, kz = ok ? gm - ky * P : 0;
+    const float cyk = g.cy[ky], syk = g.sy[ky], czk = g.cz[kz], szk = g.sz[kz];
+    coef[threadIdx.x][0] = g.gx * cyk * czk;
+    coef[threadIdx.x][1] = g.gy * syk * czk;
+    coef[threadIdx.x][2] = g.gz * cyk * szk;
+    coef[threadIdx.x][3] = (kz == 0 || 2 * kz == g.nz) ? 1.f : 2.f;
+    valid[threadIdx.x] = ok;
+  }
+  __syncthreads();
+
+  // stage 0: h_i(x) from the stress at x and x + 1 (the second row is read
+  // again by the neighbouring item and comes from the cache)
+  for (int it = threadIdx.x; it < N * TM; it += NT) {
+    const int x = it / TM, m = it - x * TM;
+    float2 h0 = make_float2(0.f, 0.f), h1 = h0, h2 = h0;
+    if (valid[m]) {
+      const int x1 = (x + 1 == N) ? 0 : x + 1;
+      const float2* p0 = S + x * colmax + 6 * m;
+      const float2* p1 = S + x1 * colmax + 6 * m;
+      const float4 a01 = ld4(p0), a23 = ld4(p0 + 2), a45 = ld4(p0 + 4);
+      const float4 b01 = ld4(p1), b23 = ld4(p1 + 2), b45 = ld4(p1 + 4);
+      const float ha = 0.5f * coef[m][0], be = coef[m][1], ga = coef[m][2];
+      // sigma components at x (a) and x + 1 (b): 0 xx, 1 yy, 2 zz, 3 xy, 4 xz, 5 yz
+      // h_i = alpha (b_ix - a_ix) / 2i + (w_i(a) + w_i(b)) / 2, w_i = beta s_iy + gamma s_iz
+      // i = x: (xx, xy, xz); i = y: (xy, yy, yz); i = z: (xz, yz, zz)
+      const float wx0r = be * (a23.z + b23.z) + ga * (a45.x + b45.x), wx0i = be * (a23.w + b23.w) + ga * (a45.y + b45.y);
+      const float wx1r = be * (a01.z + b01.z) + ga * (a45.z + b45.z), wx1i = be * (a01.w + b01.w) + ga * (a45.w + b45.w);
+      const float wx2r = be * (a45.z + b45.z) + ga * (a23.x + b23.x), wx2i = be * (a45.w + b45.w) + ga * (a23.y + b23.y);
+      h0 = make_float2(ha * (b01.y - a01.y) + 0.5f * wx0r, -ha * (b01.x - a01.x) + 0.5f * wx0i);
+      h1 = make_float2(ha * (b23.w - a23.w) + 0.5f * wx1r, -ha * (b23.z - a23.z) + 0.5f * wx1i);
+      h2 = make_float2(ha * (b45.y - a45.y) + 0.5f * wx2r, -ha * (b45.x - a45.x) + 0.5f * wx2i);
+    }
+    H[x * LH + 3 * m] = h0;
+    H[x * LH + 3 * m + 1] = h1;
+    H[x * LH + 3 * m + 2] = h2;
+  }
+  if (tile == 0 && threadIdx.x < 64) {
+    // mean stress of the line kx = 0 of mode 0 (denominator of the residual):
+    // one wave sums the six components over x
+    float acc6[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int x = threadIdx.x; x < N; x += 64) {
+      const float2* p0 = S + x * colmax;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) acc6[c] += p0[c].x;
+    }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      double v = wave_sum((double)acc6[c]);
+      if (threadIdx.x == 0) dc[r * 6 + c] = (float)v;
+    }
+  }
+  __syncthreads();
+
+  // stage 1 (forward): item (t, line), A-point DFTs over x = t + B j
+  for (int it = threadIdx.x; it < B * LH; it += NT) {
+    const int t = it / LH, l = it - t * LH;
+    float2 v[A];
+#pragma unroll
+    for (int j = 0; j < A; ++j) v[j] = H[(t + B * j) * LH + l];
+    dft_reg<A, false>(v);
+#pragma unroll
+    for (int k1 = 1; k1 < A; ++k1) v[k1] = cmul(v[k1], tw[t * k1]);
+#pragma unroll
+    for (int k1 = 0; k1 < A; ++k1) W[L::at(l, t + B * k1)] = v[k1];
+  }
+  __syncthreads();
+
+  // stage 2: item (k1, m): B-point DFTs of h, residual and Gamma0 per kx,
+  // inverse B-point DFTs of u'
+  double acc = 0.0;
+  for (int it = threadIdx.x; it < A * TM; it += NT) {
+    const int m = it / A, k1 = it - m * A;
+    if (!valid[m]) continue;
+    float2 u[3][B];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+#pragma unroll
+      for (int t = 0; t < B; ++t) u[i][t] = W[L::at(3 * m + i, t + B * k1)];
+      dft_reg<B, false>(u[i]);
+    }
+    const float al = coef[m][0], be = coef[m][1], ga = coef[m][2];
+    double e2 = 0.0;
+#pragma unroll
+    for (int k2 = 0; k2 < B; ++k2) {
+      const int kx = k1 + A * k2;
+      float2 d0 = u[0][k2], d1 = u[1][k2], d2 = u[2][k2];
+      if (DB) {
+        const float2 ph = g.phx[kx];
+        const float2* b = DB + (kx * TM + m) * 3;
+        d0 = cadd(d0, cmul(ph, b[0]));
+        d1 = cadd(d1, cmul(ph, b[1]));
+        d2 = cadd(d2, cmul(ph, b[2]));
+      }
+      e2 += (double)d0.x * d0.x + (double)d0.y * d0.y + (double)d1.x * d1.x + (double)d1.y * d1.y +
+            (double)d2.x * d2.x + (double)d2.y * d2.y;
+      const float sxk = g.sx[kx], cxk = g.cx[kx];
+      const float k0 = al * sxk;
+      const float kk1 = be * cxk;
+      const float kk2 = ga * cxk;
+      const float q2 = k0 * k0 + kk1 * kk1 + kk2 * kk2;
+      if (q2 > 0.f) {
+        const float inv = 1.0f / sqrtf(q2);
+        const float n0 = k0 * inv, n1 = kk1 * inv, n2 = kk2 * inv;
+        const float rhox = (n0 * d0.x + n1 * d1.x + n2 * d2.x) * inv, rhoy = (n0 * d0.y + n1 * d1.y + n2 * d2.y) * inv;
+        const float cA = d.cA * inv * inv, cB = d.cB * inv;
+        u[0][k2] = make_float2(cA * d0.x + cB * rhox * n0, cA * d0.y + cB * rhoy * n0);
+        u[1][k2] = make_float2(cA * d1.x + cB * rhox * n1, cA * d1.y + cB * rhoy * n1);
+        u[2][k2] = make_float2(cA * d2.x + cB * rhox * n2, cA * d2.y + cB * rhoy * n2);
+      } else {
+        u[0][k2] = u[1][k2] = u[2][k2] = make_float2(0.f, 0.f);
+      }
+    }
+    acc += (double)coef[m][3] * e2;
+    if (do_gamma) {
+      float2 twk[B];
+#pragma unroll
+      for (int t = 1; t < B; ++t) twk[t] = tw[t * k1];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        dft_reg<B, true>(u[i]);
+#pragma unroll
+        for (int t = 1; t < B; ++t) u[i][t] = cmulc(u[i][t], twk[t]);
+#pragma unroll
+        for (int t = 0; t < B; ++t) W[L::at(3 * m + i, t + B * k1)] = u[i][t];
+      }
+    }
+  }
+  const double bs = block_sum<NT>(acc, red);
+  if (threadIdx.x == 0) partials[(long long)r * ntiles + tile] = bs;
+  if (!do_gamma) return;
+  __syncthreads();
+
+  // stage 3 (inverse): item (t, line): A-point inverse DFTs, g(x = t + B j) into H
+  for (int it = threadIdx.x; it < B * LH; it += NT) {
+    const int t = it / LH, l = it - t * LH;
+    float2 v[A];
+#pragma unroll
+    for (int k1 = 0; k1 < A; ++k1) v[k1] = W[L::at(l, t + B * k1)];
+    dft_reg<A, true>(v);
+#pragma unroll
+    for (int j = 0; j < A; ++j) H[(t + B * j) * LH + l] = v[j];
+  }
+  __syncthreads();
+
+  // stage 4: item (x, m): the six strain components from g(x), g(x - 1)
+  for (int it = threadIdx.x; it < N * TM; it += NT) {
+    const int x = it / TM, m = it - x * TM;
+    if (!valid[m]) continue;
+    const float al = 0.5f * coef[m][0], be = 0.5f * coef[m][1], ga = 0.5f * coef[m][2];
+    const int xm = (x == 0) ? N - 1 : x - 1;
+    const float2 gx0 = H[x * LH + 3 * m], gy0 = H[x * LH + 3 * m + 1], gz0 = H[x * LH + 3 * m + 2];
+    const float2 gx1 = H[xm * LH + 3 * m], gy1 = H[xm * LH + 3 * m + 1], gz1 = H[xm * LH + 3 * m + 2];
+    // D_j = (g_j(x) - g_j(x-1)) / 2i = ((dy)/2, -(dx)/2); Av_j = (g_j(x) + g_j(x-1)) / 2
+    const float2 Dx = make_float2(0.5f * (gx0.y - gx1.y), -0.5f * (gx0.x - gx1.x));
+    const float2 Dy = make_float2(0.5f * (gy0.y - gy1.y), -0.5f * (gy0.x - gy1.x));
+    const float2 Dz = make_float2(0.5f * (gz0.y - gz1.y), -0.5f * (gz0.x - gz1.x));
+    const float2 Ax = make_float2(gx0.x + gx1.x, gx0.y + gx1.y);  // 2 Av
+    const float2 Ay = make_float2(gy0.x + gy1.x, gy0.y + gy1.y);
+    const float2 Az = make_float2(gz0.x + gz1.x, gz0.y + gz1.y);
+    // r_xx = alpha D_x, r_yy = beta Av_y, r_zz = gamma Av_z,
+    // r_xy = (alpha D_y + beta Av_x) / 2, r_xz = (alpha D_z + gamma Av_x) / 2,
+    // r_yz = (beta Av_z + gamma Av_y) / 2   (al, be, ga hold half the coefficients)
+    const float4 o01 = make_float4(2.f * al * Dx.x, 2.f * al * Dx.y, be * Ay.x, be * Ay.y);
+    const float4 o23 = make_float4(ga * Az.x, ga * Az.y, al * Dy.x + 0.5f * be * Ax.x, al * Dy.y + 0.5f * be * Ax.y);
+    const float4 o45 = make_float4(al * Dz.x + 0.5f * ga * Ax.x, al * Dz.y + 0.5f * ga * Ax.y,
+                                   0.5f * (be * Az.x + ga * Ay.x), 0.5f * (be * Az.y + ga * Ay.y));
+    float4* o = reinterpret_cast<float4*>(S + x * colmax + 6 * m);
+    o[0] = o01;
+    o[1] = o23;
+    o[2] = o45;
+  }
+}
+
+}  // namespace lsk

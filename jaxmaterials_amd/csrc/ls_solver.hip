@@ -46,6 +46,7 @@ struct HostAxis {
   double2* d_tw64 = nullptr;
   float* d_s = nullptr;
   float* d_c = nullptr;
+  float2* d_ph = nullptr;  // e^{+i pi k / n}
 };
 
 std::vector<int> factorise(int n) {
@@ -116,7 +117,7 @@ struct Tile {
   int E = 0;      // register elements per thread (LDS engine)
   int NT = 0;     // threads per block
   size_t lds = 0;
-  int reg = 0;    // 1: register two-stage kernel (k_ypass_r / k_xpass_r)
+  int reg = 0;    // 1: register two-stage kernel (k_ypass_r / k_xpass_r); 2: k_xpass_h
 };
 
 // Register two-stage kernels compiled for the axis lengths of the workloads
@@ -131,16 +132,25 @@ struct Tile {
   F(96, 12, 8, 4, 192) F(96, 12, 8, 8, 384) F(64, 8, 8, 4, 192) F(64, 8, 8, 8, 384)            \
   F(48, 6, 8, 4, 192) F(48, 6, 8, 8, 384) F(32, 8, 4, 8, 192) F(32, 8, 4, 16, 384)
 
+// x pass with three-line transforms (k_xpass_h): (n, A, B, TM, NT)
+#define LS_REG_H_LIST(F)                                                                        \
+  F(256, 32, 8, 4, 256) F(256, 16, 16, 4, 256) F(256, 32, 8, 4, 128) F(128, 16, 8, 8, 256)   \
+  F(128, 16, 8, 4, 256) F(96, 12, 8, 8, 256) F(96, 12, 8, 4, 128) F(64, 8, 8, 8, 256)          \
+  F(64, 8, 8, 16, 512) F(48, 6, 8, 16, 256) F(48, 6, 8, 8, 256) F(32, 8, 4, 16, 256)          \
+  F(32, 8, 4, 32, 512)
+
 struct RegCfg {
   int n, A, B, lines, NT;
 };
 #define LS_AS_CFG(NV, AV, BV, LV, NTV) {NV, AV, BV, LV, NTV},
 const RegCfg kRegY[] = {LS_REG_Y_LIST(LS_AS_CFG)};
 const RegCfg kRegX[] = {LS_REG_X_LIST(LS_AS_CFG)};
+const RegCfg kRegH[] = {LS_REG_H_LIST(LS_AS_CFG)};
 #undef LS_AS_CFG
 
-// Register-kernel tiles of an axis; lines_per_unit = 1 (y columns) or 6 (x modes).
-std::vector<Tile> reg_tiles(const RegCfg* list, size_t count, int n, int lines_per_unit) {
+// Register-kernel tiles of an axis; lines_per_unit = 1 (y columns) or 6 (x
+// modes); kind 2 = the three-line x pass (LDS: 3 TM (n + Np) complex).
+std::vector<Tile> reg_tiles(const RegCfg* list, size_t count, int n, int lines_per_unit, int kind = 1) {
   std::vector<Tile> out;
   for (size_t i = 0; i < count; ++i) {
     const RegCfg& c = list[i];
@@ -148,9 +158,11 @@ std::vector<Tile> reg_tiles(const RegCfg* list, size_t count, int n, int lines_p
     Tile t;
     t.lines = c.lines;
     t.NT = c.NT;
-    t.reg = 1;
+    t.reg = kind;
+    t.E = c.A;  // distinguishes the (A, B) splits of one length
     const int np = c.n + c.n / c.B + 1;
-    t.lds = (size_t)c.lines * lines_per_unit * np * sizeof(float2);
+    t.lds = kind == 2 ? (size_t)3 * c.lines * (c.n + np) * sizeof(float2)
+                      : (size_t)c.lines * lines_per_unit * np * sizeof(float2);
     out.push_back(t);
   }
   return out;
@@ -253,7 +265,16 @@ int build_axis(HostAxis& a, int n) {
   }
   s[0] = 0.f;
   if (n % 2 == 0) c[n / 2] = 0.f;
-  if (dalloc(&a.d_tw, n) || dalloc(&a.d_tw64, n) || dalloc(&a.d_s, n) || dalloc(&a.d_c, n)) return -1;
+  std::vector<float2> ph(n);
+  for (int k = 0; k < n; ++k) {
+    const long double ang = 3.141592653589793238462643383279502884L * (long double)k / (long double)n;
+    ph[k] = make_float2((float)cosl(ang), (float)sinl(ang));
+  }
+  if (n % 2 == 0) ph[n / 2] = make_float2(0.f, 1.f);
+  if (dalloc(&a.d_tw, n) || dalloc(&a.d_tw64, n) || dalloc(&a.d_s, n) || dalloc(&a.d_c, n) ||
+      dalloc(&a.d_ph, n))
+    return -1;
+  LS_HIP(hipMemcpy(a.d_ph, ph.data(), n * sizeof(float2), hipMemcpyHostToDevice));
   LS_HIP(hipMemcpy(a.d_tw, tw.data(), n * sizeof(float2), hipMemcpyHostToDevice));
   LS_HIP(hipMemcpy(a.d_tw64, tw64.data(), n * sizeof(double2), hipMemcpyHostToDevice));
   LS_HIP(hipMemcpy(a.d_s, s.data(), n * sizeof(float), hipMemcpyHostToDevice));
@@ -334,7 +355,7 @@ void ypass(ls_solver* s, bool inv) {
   const size_t lds = s->ty.lds;
   if (s->ty.reg) {
 #define LS_YR(NV, AV, BV, LV, NTV)                                                                      \
-  else if (s->ny == NV && s->ty.lines == LV && s->ty.NT == NTV) {                                       \
+  else if (s->ny == NV && s->ty.lines == LV && s->ty.NT == NTV && s->ty.E == AV) {                     \
     if (inv)                                                                                            \
       k_ypass_r<AV, BV, LV, NTV, true><<<grid, NTV, lds, s->stream>>>(s->gd, s->d_st, s->spec, s->ntiles_y);  \
     else                                                                                                \
@@ -363,9 +384,23 @@ void xpass(ls_solver* s, int do_gamma, bool with_base = true) {
   const unsigned grid = (unsigned)s->nblk_x * s->nrhs;
   const size_t lds = s->tx.lds;
   const float2* db = with_base ? s->dbase : nullptr;
+  if (s->tx.reg == 2) {
+#define LS_XH(NV, AV, BV, LV, NTV)                                                                        \
+  else if (s->nx == NV && s->tx.lines == LV && s->tx.NT == NTV && s->tx.E == AV) {                       \
+    k_xpass_h<AV, BV, LV, NTV><<<grid, NTV, lds, s->stream>>>(s->gd, s->d_rhs, s->d_st, s->spec, db, s->partials, \
+                                                              s->dc, s->nblk_x, s->dl.per_rhs, do_gamma);   \
+  }
+    if (false) {
+    }
+    LS_REG_H_LIST(LS_XH)
+    else fprintf(stderr, "liblippmannschwinger: no three-line x kernel for nx=%d TM=%d\n", s->nx, s->tx.lines);
+#undef LS_XH
+    mark(s, 2);
+    return;
+  }
   if (s->tx.reg) {
 #define LS_XR(NV, AV, BV, LV, NTV)                                                                        \
-  else if (s->nx == NV && s->tx.lines == LV && s->tx.NT == NTV) {                                         \
+  else if (s->nx == NV && s->tx.lines == LV && s->tx.NT == NTV && s->tx.E == AV) {                       \
     k_xpass_r<AV, BV, LV, NTV><<<grid, NTV, lds, s->stream>>>(s->gd, s->d_rhs, s->d_st, s->spec, db, s->partials, \
                                                               s->dc, s->nblk_x, s->dl.per_rhs, do_gamma);   \
   }
@@ -465,6 +500,7 @@ void free_solver(ls_solver* s) {
     (void)hipFree(a.d_tw64);
     (void)hipFree(a.d_s);
     (void)hipFree(a.d_c);
+    (void)hipFree(a.d_ph);
   }
   void* dev[] = {s->spec,    s->spec64,  s->base,    s->dbase,    s->partials, s->dc,       s->d_rhs,
                  s->d_st,    s->d_sel,   s->part6,   s->d_mean6,  s->d_mm,     s->aaG,      s->aaR,
@@ -547,8 +583,12 @@ int create_impl(ls_solver* s, const int* voxels, const float* extents, int nrhs,
   if (env_int("LS_REG_FFT", 1)) {
     const std::vector<Tile> ry = reg_tiles(kRegY, sizeof(kRegY) / sizeof(kRegY[0]), s->ny, 1);
     const std::vector<Tile> rx = reg_tiles(kRegX, sizeof(kRegX) / sizeof(kRegX[0]), s->nx, 6);
+    const std::vector<Tile> rh = reg_tiles(kRegH, sizeof(kRegH) / sizeof(kRegH[0]), s->nx, 6, 2);
     if (!ry.empty()) s->ty = ry[0];
-    if (!rx.empty()) s->tx = rx[0];
+    if (!rh.empty())
+      s->tx = rh[0];
+    else if (!rx.empty())
+      s->tx = rx[0];
   }
   if (s->tx.lines > kMaxTm) {
     fprintf(stderr, "liblippmannschwinger: x tile of %d modes exceeds %d\n", s->tx.lines, kMaxTm);
@@ -575,6 +615,7 @@ int create_impl(ls_solver* s, const int* voxels, const float* extents, int nrhs,
   g.cy = s->ax[1].d_c;
   g.sz = s->ax[2].d_s;
   g.cz = s->ax[2].d_c;
+  g.phx = s->ax[0].d_ph;
   g.px = to_dev(s->ax[0]);
   g.py = to_dev(s->ax[1]);
   g.pz = to_dev(s->ax[2]);
@@ -1010,6 +1051,7 @@ int autotune_impl(ls_solver* s, const float* lam, const float* mu, float* eps, i
   LS_HIP(hipMemsetAsync(s->dbase, 0, (size_t)s->nrhs * s->dl.per_rhs * sizeof(float2), s->stream));
   LS_HIP(hipMemsetAsync(s->dc, 0, (size_t)s->nrhs * 6 * sizeof(float), s->stream));
   const int lsz = s->gd.lsz;
+  const bool verbose_tune = std::getenv("LS_TUNE_VERBOSE") != nullptr;
   auto sweep = [&](const char* name, Tile& cur, const std::vector<Tile>& cands, auto&& launch) -> int {
     Tile best = cur;
     float best_ms = -1.f;
@@ -1019,6 +1061,9 @@ int autotune_impl(ls_solver* s, const float* lam, const float* mu, float* eps, i
       apply_tiles(s);
       const float ms = time_launches(s, reps, launch);
       if (ms < 0.f) return -1;
+      if (verbose_tune)
+        fprintf(stderr, "  tune %s: %s lines %d E %d NT %d lds %zu: %.4f ms\n", name,
+                t.reg == 2 ? "three-line" : (t.reg ? "register" : "lds"), t.lines, t.E, t.NT, t.lds, ms);
       if (best_ms < 0.f || ms < best_ms) {
         best_ms = ms;
         best = t;
@@ -1027,7 +1072,8 @@ int autotune_impl(ls_solver* s, const float* lam, const float* mu, float* eps, i
     cur = best;
     apply_tiles(s);
     snprintf(buf, sizeof(buf), "%s: %s %d lines E %d NT %d %.3f ms (%zu candidates); ", name,
-             best.reg ? "register" : "lds", best.lines, best.E, best.NT, best_ms, cands.size());
+             best.reg == 2 ? "three-line" : (best.reg ? "register" : "lds"), best.lines, best.E, best.NT, best_ms,
+             cands.size());
     rep += buf;
     return 0;
   };
@@ -1039,6 +1085,7 @@ int autotune_impl(ls_solver* s, const float* lam, const float* mu, float* eps, i
   std::vector<Tile> cx = tile_candidates(s->ax[0], wants(wx), 6, (size_t)6 * s->nx * sizeof(float2), kMaxTm);
   for (const Tile& t : reg_tiles(kRegY, sizeof(kRegY) / sizeof(kRegY[0]), s->ny, 1)) cy.push_back(t);
   for (const Tile& t : reg_tiles(kRegX, sizeof(kRegX) / sizeof(kRegX[0]), s->nx, 6)) cx.push_back(t);
+  for (const Tile& t : reg_tiles(kRegH, sizeof(kRegH) / sizeof(kRegH[0]), s->nx, 6, 2)) cx.push_back(t);
   if (sweep("z", s->tz, tile_candidates(s->ax[2], wants(wz), 3, (size_t)3 * lsz * sizeof(float2), 1 << 20),
             [&] { zline<Z_UPD>(s, lam, mu, eps); }) ||
       sweep("y", s->ty, cy, [&] { ypass(s, true); }) || sweep("x", s->tx, cx, [&] { xpass(s, 1); }))
