@@ -30,30 +30,37 @@ def model_bytes(kind, N, M, iters, depth, nrebase=0):
 
     One HBM round trip of the z-half spectrum (48 M bytes per load case) per
     pass and direction; per load case and iteration j (f_j = min(j, depth+1)
-    filled Anderson slots):
-      z plain           : spectrum 96 M + strain 48 N (+ Lame 8 N shared by the batch)
-      z Anderson resid. : spectrum 48 M + strain 24 N + r, g written 48 N + r history (f_j - 1) 24 N
-      z Anderson mix    : g history f_j 24 N + strain 24 N + spectrum 48 M (+ Lame 8 N shared)
-      y (inverse + forward) : 2 x 96 M;  x : 96 M (setup, every check, every re-evaluation)
+    filled Anderson slots when the z pass of iteration j runs):
+      z plain    : spectrum 96 M + strain 48 N (+ Lame 8 N shared by the batch)
+      z Anderson : spectrum 96 M + strain 48 N + newest g written 24 N
+                   + older g read (f_j - 1) 24 N (+ Lame 8 N shared)
+      y (inverse + forward) : 2 x 96 M
+      x : 96 M per evaluation (setup, every check, every re-evaluation); with
+          Anderson also the residual's three-complex-per-mode form y written 24 M
+      aa_gram : Anderson Gram row, the f newest y read, f * 24 M,
+          f = min(e + 1, depth + 1) at the e-th evaluation
     """
     its = [int(i) for i in iters]
     m1 = depth + 1
     nb = max(its) if its else 0
     if kind == "z":
         return sum(k * (96 * M + 48 * N) for k in its) + nb * 8 * N if depth == 0 else 0
-    if kind == "z_aa_residual":
-        return sum(48 * M + 48 * N + 24 * N + 24 * N * (min(j, m1) - 1) for k in its for j in range(1, k + 1))
-    if kind == "z_aa_mix":
-        return sum(24 * N * min(j, m1) + 24 * N + 48 * M for k in its for j in range(1, k + 1)) + nb * 8 * N
+    if kind == "z_aa":
+        return sum(96 * M + 72 * N + 24 * N * (min(j, m1) - 1) for k in its for j in range(1, k + 1)) + nb * 8 * N
     if kind == "y":
         return sum(2 * k for k in its) * 96 * M
     if kind == "x":
-        return (sum(k + 1 for k in its) + nrebase * len(its)) * 96 * M
+        b = (sum(k + 1 for k in its) + nrebase * len(its)) * 96 * M
+        if depth > 0:
+            b += sum(k + 1 for k in its) * 24 * M
+        return b
+    if kind == "aa_gram":
+        return sum(24 * M * min(e + 1, m1) for k in its for e in range(k + 1)) if depth > 0 else 0
     return 0
 
 
-KERNEL_OF = {"z": "k_zline<Z_UPD>", "z_aa_residual": "k_zline<Z_AAINV>", "z_aa_mix": "k_zline<Z_AAMIX>",
-             "y": "k_ypass", "x": "k_xpass"}
+KERNEL_OF = {"z": "k_zline<Z_UPD>", "z_aa": "k_zline<Z_AA>", "y": "k_ypass_r", "x": "k_xpass_h",
+             "aa_gram": "k_aa_gram"}
 
 
 def measured_traffic(kind, n, method):
@@ -205,7 +212,7 @@ def main():
         C = out["sigma_mean"]
         sec = float(np.mean(times))
         kinds = {}
-        for name in ("z", "z_aa_residual", "z_aa_mix", "y", "x"):
+        for name in ("z", "z_aa", "y", "x", "aa_gram"):
             if cnt_kind[name] == 0:
                 continue
             b = model_bytes(name, N, M, iters, depth, nreb)

@@ -24,11 +24,14 @@
 //                the six Voigt components of one mode adjacent (48 B)
 //   dbase      : [R][nx][ny*P][3] complex fp32, divergence of the base stress
 // One FP32 iteration is four passes over the spectrum:
-//   k_zline   : per (x,y) line: C2R along z -> delta += r -> Hooke -> R2C along z
+//   k_zline   : per (x,y) line: C2R along z -> delta update -> Hooke -> R2C along z
 //   k_ypass   : forward FFT along y
 //   k_xpass   : forward FFT along x -> divergence, residual, Gamma0 -> inverse FFT along x
 //   k_ypass   : inverse FFT along y
 // followed by k_check (deterministic FP64 residual reduction, stop test).
+// With Anderson acceleration the x pass also records the residual's spectral
+// representation and its FP64 Gram row, so the weights are known before the
+// z pass, which then updates, mixes, applies Hooke and transforms in one go.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -59,15 +62,26 @@ struct GridDev {
 // Layout of d_base matched to the tiles of the FP32 x pass: [tile][x][m][3]
 // with tile = gm / Tm and m = gm % Tm, so each x-pass block reads one
 // contiguous range.
+// Two orders inside a tile of nx * Tm * 3 complex entries:
+//  * LDS-engine x pass (hA = 0): [x][m][3];
+//  * three-line x pass (hA = A, nx = A B, x = k1 + A k2): [k2][3][m][k1], so
+//    the threads of its stage 2 (consecutive k1) read consecutive entries.
+// index() gives component 0, cstride() the step between the components.
 struct DbLayout {
   FastDiv f_Tm;        // modes per tile of the FP32 x pass
+  FastDiv f_A;         // three-line x pass: A
+  int hA;              // 0 or A
   int nx;
   long long per_rhs;  // complex entries per load case
   __device__ __forceinline__ long long index(int x, int gm) const {
     const int t = fdiv(gm, f_Tm);
-    const int m = gm - t * (int)f_Tm.d;
-    return (((long long)t * nx + x) * (long long)f_Tm.d + m) * 3;
+    const int Tm = (int)f_Tm.d;
+    const int m = gm - t * Tm;
+    if (hA == 0) return (((long long)t * nx + x) * (long long)Tm + m) * 3;
+    const int k2 = fdiv(x, f_A), k1 = x - k2 * hA;
+    return (long long)t * nx * Tm * 3 + ((long long)k2 * 3 * Tm + m) * hA + k1;
   }
+  __device__ __forceinline__ int cstride() const { return hA == 0 ? 1 : (int)f_Tm.d * hA; }
 };
 
 struct RhsDesc {
@@ -94,21 +108,42 @@ enum ZMode : int {
   Z_UPD = 2,    // C2R -> delta += r -> sigma = C delta -> R2C
   Z_SET = 3,    // C2R -> eps = eps_bar - r (end of a projection; no forward transform)
   Z_PROJ = 4,   // eps read, sigma = C0 (eps - eps_bar), R2C  (start of a projection)
-  Z_AAINV = 5,  // C2R -> r; history slot <- r, g = delta + r; FP64 Gram-row partials (no R2C)
-  Z_AAMIX = 6,  // delta = sum_s alpha_s g^(s) -> sigma = C delta -> R2C
+  Z_AA = 5,     // C2R -> r; g = delta + r into the newest slot; delta = sum_s alpha_s g^(s) -> C delta -> R2C
 };
 
-// Anderson history (lippmann_schwinger.py:177-231, doc/linear_elasticity.tex:335-370),
-// stored per load case as ring buffers of g^(s) = delta^(s) + r^(s) and r^(s).
+// Anderson history (lippmann_schwinger.py:177-231, doc/linear_elasticity.tex:335-370).
+// Per load case a ring of g^(s) = delta^(s) + r^(s) in real space (the
+// mixing operands) and a ring of three complex numbers per mode from which
+// the Gram row <r, r^(s)>_w is a plain dot product.  With
+// r_hat = sym(xi~ (x) u) (u from the x pass), Parseval and the Hermitian
+// weight w_k of the half spectrum,
+//   <r_hat, r_hat'>_w = w_k (|xi~|^2 u . conj(u') + (xi~ . u) conj(xi~ . u')) / 2
+//                     = Re(y . conj(y')),
+//   y = sqrt(w_k / 2) |xi~| (u + (sqrt(2) - 1) (n . u) n),  n = xi~ / |xi~|
+// (y = sqrt(w_k/2) Q^{1/2} u with Q = |xi~|^2 I + xi~ xi~^T), so the x pass
+// stores y and k_aa_gram streams the ring.  Entries outside the grid's modes
+// stay zero.
 struct AAParams {
   float* G;             // [R][m1][6][N]
-  float* Rh;            // [R][m1][6][N]
+  float2* U;            // [R][m1][u_per_rhs] complex (y per mode, three per mode)
+  long long u_per_rhs;  // complex entries per slot (= DbLayout::per_rhs)
   const double* alpha;  // [R][m1] mixing weights by slot
-  double* dots;         // [R][nblk][m1] Gram-row partials by age
-  const int* head;      // [R] slot of the newest entry for this step
+  const int* head;      // [R] slot of the newest entry
   const int* filled;    // [R] number of filled slots including the newest
   int m1;               // depth + 1
+  int record;           // x pass: store y of the newest residual
 };
+
+// y of one mode (see AAParams); q2 = |xi~|^2 > 0, inv = 1 / |xi~|
+__device__ __forceinline__ void aa_y(float wk, float q2, float inv, float n0, float n1, float n2, float2 u0,
+                                     float2 u1, float2 u2, float2& y0, float2& y1, float2& y2) {
+  constexpr float c = 0.41421356237309504880f;  // sqrt(2) - 1
+  const float sc = sqrtf(0.5f * wk) * q2 * inv;  // sqrt(w/2) |xi~|
+  const float pr = c * (n0 * u0.x + n1 * u1.x + n2 * u2.x), pi = c * (n0 * u0.y + n1 * u1.y + n2 * u2.y);
+  y0 = make_float2(sc * (u0.x + pr * n0), sc * (u0.y + pi * n0));
+  y1 = make_float2(sc * (u1.x + pr * n1), sc * (u1.y + pi * n1));
+  y2 = make_float2(sc * (u2.x + pr * n2), sc * (u2.y + pi * n2));
+}
 
 __device__ __forceinline__ float4 ld4(const float2* p) { return *reinterpret_cast<const float4*>(p); }
 
@@ -203,7 +238,7 @@ __global__ void __launch_bounds__(NT) k_zline(GridDev g, const RhsDesc* __restri
   const int tid = threadIdx.x;
   const FastDiv f_none = g.f_nz;  // unused by the line-contiguous layout
 
-  if constexpr (MODE == Z_UPD || MODE == Z_SET || MODE == Z_AAINV) {
+  if constexpr (MODE == Z_UPD || MODE == Z_SET || MODE == Z_AA) {
     // gather the half spectra into the paired full-length lines (C2R semantics:
     // imaginary parts of the k = 0 and k = nz/2 coefficients are ignored)
     const int items = L * P * 3;
@@ -226,59 +261,6 @@ __global__ void __launch_bounds__(NT) k_zline(GridDev g, const RhsDesc* __restri
     fft_lines<true, false, true, E, NT>(lds, g.pz, f_none, 3 * L, 1, ls);
   }
 
-  if constexpr (MODE == Z_AAINV) {
-    // store r and g = delta + r in the newest slot; Gram-row partials by age
-    const int m1 = aa.m1;
-    const int h = aa.head[r];
-    const int f = aa.filled[r];
-    const long long slot_stride = 6 * N;
-    float* Gb = aa.G + (long long)r * m1 * slot_stride;
-    float* Rb = aa.Rh + (long long)r * m1 * slot_stride;
-    double acc[kMaxDepth1];
-#pragma unroll
-    for (int s = 0; s < kMaxDepth1; ++s) acc[s] = 0.0;
-    const int items = L * nz;
-    for (int e = tid; e < items; e += NT) {
-      const int l = fdiv(e, g.f_nz), j = e - l * nz;
-      const long long v = (long long)(line0 + l) * nz + j;
-      float rr[6];
-#pragma unroll
-      for (int p = 0; p < 3; ++p) {
-        const float2 z = lds[lds_idx<true>(l * 3 + p, j, 1, ls)];
-        rr[2 * p] = z.x;
-        rr[2 * p + 1] = z.y;
-      }
-      float* Rn = Rb + h * slot_stride;
-      float* Gn = Gb + h * slot_stride;
-      double a0 = 0.0;
-#pragma unroll
-      for (int c = 0; c < 6; ++c) {
-        const float ec = E_[c * N + v];
-        Rn[c * N + v] = rr[c];
-        Gn[c * N + v] = ec + rr[c];
-        a0 += (c < 3 ? 1.0 : 2.0) * (double)rr[c] * rr[c];
-      }
-      acc[0] += a0;
-#pragma unroll
-      for (int s = 1; s < kMaxDepth1; ++s) {
-        if (s < f) {
-          int sl = h - s;
-          if (sl < 0) sl += m1;
-          const float* Rs = Rb + sl * slot_stride;
-          double a = 0.0;
-#pragma unroll
-          for (int c = 0; c < 6; ++c) a += (c < 3 ? 1.0 : 2.0) * (double)rr[c] * Rs[c * N + v];
-          acc[s] += a;
-        }
-      }
-    }
-    for (int s = 0; s < m1; ++s) {
-      const double bs = block_sum<NT>(acc[s], red);
-      if (tid == 0) aa.dots[((long long)r * ntiles + tile) * m1 + s] = (s < f) ? bs : 0.0;
-    }
-    return;
-  }
-
   // pointwise: update / Hooke, write sigma pairs into LDS
   {
     const int items = L * nz;
@@ -289,17 +271,31 @@ __global__ void __launch_bounds__(NT) k_zline(GridDev g, const RhsDesc* __restri
       if constexpr (MODE == Z_PROJ) {
 #pragma unroll
         for (int c = 0; c < 6; ++c) ev[c] = E_[c * N + v];
-      } else if constexpr (MODE == Z_AAMIX) {
+      } else if constexpr (MODE == Z_AA) {
+        // r from the inverse transform; g = delta + r into the newest slot;
+        // delta = sum_s alpha_s g^(s) (the newest from registers)
         const int m1 = aa.m1;
         const int f = aa.filled[r];
         const int h = aa.head[r];
         const long long slot_stride = 6 * N;
-        const float* Gb = aa.G + (long long)r * m1 * slot_stride;
+        float* Gb = aa.G + (long long)r * m1 * slot_stride;
         const double* al = aa.alpha + (long long)r * m1;
+        float gn[6];
 #pragma unroll
-        for (int c = 0; c < 6; ++c) ev[c] = 0.f;
-        for (int s = 0; s < f; ++s) {
-          int sl = h - s;
+        for (int p = 0; p < 3; ++p) {
+          const float2 z = lds[lds_idx<true>(l * 3 + p, j, 1, ls)];
+          gn[2 * p] = E_[(2 * p) * N + v] + z.x;
+          gn[2 * p + 1] = E_[(2 * p + 1) * N + v] + z.y;
+        }
+        float* Gh = Gb + h * slot_stride;
+        const float a0 = (float)al[h];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+          Gh[c * N + v] = gn[c];
+          ev[c] = a0 * gn[c];
+        }
+        for (int s2 = 1; s2 < f; ++s2) {
+          int sl = h - s2;
           if (sl < 0) sl += m1;
           const float a = (float)al[sl];
           const float* Gs = Gb + sl * slot_stride;
@@ -409,14 +405,15 @@ __device__ __forceinline__ void xi_tilde(const GridDev& g, int kx, int ky, int k
 // the divergence: n = xi~/|xi~|, v = d/|xi~| (= sigma_hat n), rho = n . v,
 //   r_hat = cA (n0 v0, n1 v1, n2 v2, (n0 v1 + n1 v0)/2, (n0 v2 + n2 v0)/2,
 //               (n1 v2 + n2 v1)/2) + cB rho (n0^2, n1^2, n2^2, n0 n1, n0 n2, n1 n2)
-template <int E, int NT>
+template <int E, int NT, bool REC>
 __global__ void __launch_bounds__(NT) k_xpass(GridDev g, const RhsDesc* __restrict__ rhs,
                                               const RhsState* __restrict__ st, float2* __restrict__ spec,
                                               const float2* __restrict__ dbase, double* __restrict__ partials,
                                               float* __restrict__ dc, FastDiv f_Tm, FastDiv f_cols, int ntiles,
-                                              DbLayout dl, int do_gamma) {
+                                              DbLayout dl, int do_gamma, AAParams aa) {
   extern __shared__ float2 lds[];
   __shared__ double red[NT / 64];
+  const bool rec = REC && do_gamma;
   const int Tm = (int)f_Tm.d;
   const int lin = xcd_remap(blockIdx.x, gridDim.x);
   const int r = lin / ntiles;
@@ -497,9 +494,23 @@ __global__ void __launch_bounds__(NT) k_xpass(GridDev g, const RhsDesc* __restri
                            h * (n0 * v2.y + n2 * v0.y) + cB * rho.y * n0 * n2);
         o[5] = make_float2(h * (n1 * v2.x + n2 * v1.x) + cB * rho.x * n1 * n2,
                            h * (n1 * v2.y + n2 * v1.y) + cB * rho.y * n1 * n2);
+        if (rec) {
+          // u = (cA v + cB rho n) / |xi~|, recorded as y of e^{+i pi kx/nx} u
+          const float2 ph = g.phx[x];
+          const float2 u0 = cmul(ph, make_float2((cA * v0.x + cB * rho.x * n0) * inv, (cA * v0.y + cB * rho.y * n0) * inv));
+          const float2 u1 = cmul(ph, make_float2((cA * v1.x + cB * rho.x * n1) * inv, (cA * v1.y + cB * rho.y * n1) * inv));
+          const float2 u2 = cmul(ph, make_float2((cA * v2.x + cB * rho.x * n2) * inv, (cA * v2.y + cB * rho.y * n2) * inv));
+          float2* Un = aa.U + ((long long)r * aa.m1 + aa.head[r]) * aa.u_per_rhs + dl.index(x, gm);
+          aa_y((float)wk, q2, inv, n0, n1, n2, u0, u1, u2, Un[0], Un[1], Un[2]);
+        }
       } else {
 #pragma unroll
         for (int c = 0; c < 6; ++c) o[c] = make_float2(0.f, 0.f);
+        if (rec) {
+          const long long ui = dl.index(x, gm);
+          float2* Un = aa.U + ((long long)r * aa.m1 + aa.head[r]) * aa.u_per_rhs + ui;
+          Un[0] = Un[1] = Un[2] = make_float2(0.f, 0.f);
+        }
       }
 #pragma unroll
       for (int c = 0; c < 6; ++c) t[c] = o[c];
@@ -597,12 +608,13 @@ __global__ void __launch_bounds__(NT) k_base_x(GridDev g, int r, const double2* 
     xi_tilde(g, x, ky, kz, f0, f1, f2);
     const double k0 = f0, k1 = f1, k2 = f2;
     float2* o = DB + dl.index(x, gm);
+    const int cs = dl.cstride();
     o[0] = make_float2((float)(k0 * s[0].x + k1 * s[3].x + k2 * s[4].x),
                        (float)(k0 * s[0].y + k1 * s[3].y + k2 * s[4].y));
-    o[1] = make_float2((float)(k0 * s[3].x + k1 * s[1].x + k2 * s[5].x),
-                       (float)(k0 * s[3].y + k1 * s[1].y + k2 * s[5].y));
-    o[2] = make_float2((float)(k0 * s[4].x + k1 * s[5].x + k2 * s[2].x),
-                       (float)(k0 * s[4].y + k1 * s[5].y + k2 * s[2].y));
+    o[cs] = make_float2((float)(k0 * s[3].x + k1 * s[1].x + k2 * s[5].x),
+                        (float)(k0 * s[3].y + k1 * s[1].y + k2 * s[5].y));
+    o[2 * cs] = make_float2((float)(k0 * s[4].x + k1 * s[5].x + k2 * s[2].x),
+                            (float)(k0 * s[4].y + k1 * s[5].y + k2 * s[2].y));
   }
 }
 
@@ -793,11 +805,48 @@ __global__ void __launch_bounds__(kThreads) k_minmax(const float* __restrict__ l
   }
 }
 
-// Anderson step per load case (lippmann_schwinger.py:217-228): reduce the new
-// Gram row <r, r^(s)>_w (FP64, fixed order), shift the Gram matrix by one age,
-// solve A v = u on the filled block by Gaussian elimination with partial
-// pivoting in FP64, alpha = v / (u . v).  A singular or non-finite system falls
-// back to the plain step alpha = e_0.  One block per load case.
+// Gram row of the newest residual of every active load case:
+// dots[r][blk][s] = sum over this block's chunk of Re(y_new . conj(y^(s))),
+// s = age (0 = newest), FP64 products and sums in a fixed order.  Grid
+// (nblk, R), `chunk` complex entries per block.
+__global__ void __launch_bounds__(kThreads) k_aa_gram(const RhsState* __restrict__ st, AAParams aa,
+                                                      double* __restrict__ dots, long long chunk) {
+  __shared__ double red[kThreads / 64];
+  const int r = blockIdx.y;
+  if (!st[r].active) return;
+  const int hd = aa.head[r], f = aa.filled[r];
+  const float4* base = reinterpret_cast<const float4*>(aa.U + (long long)r * aa.m1 * aa.u_per_rhs);
+  const long long slot4 = aa.u_per_rhs / 2;
+  const long long i0 = (long long)blockIdx.x * chunk / 2, i1 = min(slot4, i0 + chunk / 2);
+  const float4* yn = base + hd * slot4;
+  double acc[kMaxDepth1];
+#pragma unroll
+  for (int q = 0; q < kMaxDepth1; ++q) acc[q] = 0.0;
+  for (long long i = i0 + threadIdx.x; i < i1; i += kThreads) {
+    const float4 a = yn[i];
+#pragma unroll
+    for (int q = 0; q < kMaxDepth1; ++q) {
+      if (q < f) {
+        int sl = hd - q;
+        if (sl < 0) sl += aa.m1;
+        const float4 b = q == 0 ? a : base[sl * slot4 + i];
+        acc[q] += (double)a.x * b.x + (double)a.y * b.y + (double)a.z * b.z + (double)a.w * b.w;
+      }
+    }
+  }
+  for (int q = 0; q < aa.m1; ++q) {
+    const double b = block_sum(acc[q], red);
+    if (threadIdx.x == 0) dots[((long long)r * gridDim.x + blockIdx.x) * aa.m1 + q] = (q < f) ? b : 0.0;
+  }
+}
+
+// Anderson weights per load case (lippmann_schwinger.py:217-228): reduce the
+// new Gram row <r, r^(s)>_w (FP64, fixed order) into row and column 0 of the
+// Gram matrix (k_aa_advance has shifted the older ages), solve A v = u on the
+// filled block by Gaussian elimination with partial pivoting in FP64,
+// alpha = v / (u . v).  A singular or non-finite system falls back to the
+// plain step alpha = e_0.  One block per load case.  Running it again after
+// the newest row was re-evaluated (a rebase) replaces that row.
 __global__ void __launch_bounds__(kThreads) k_aa_solve(const RhsState* __restrict__ st,
                                                        const double* __restrict__ dots, int nblk, int m1,
                                                        double* __restrict__ gram, double* __restrict__ alpha,
@@ -816,8 +865,6 @@ __global__ void __launch_bounds__(kThreads) k_aa_solve(const RhsState* __restric
   }
   if (threadIdx.x != 0) return;
   double* A = gram + (long long)r * m1 * m1;
-  for (int i = m1 - 1; i >= 1; --i)
-    for (int j = m1 - 1; j >= 1; --j) A[i * m1 + j] = A[(i - 1) * m1 + (j - 1)];
   for (int s = 0; s < m1; ++s) {
     A[s] = row[s];
     A[s * m1] = row[s];
@@ -869,13 +916,18 @@ __global__ void __launch_bounds__(kThreads) k_aa_solve(const RhsState* __restric
   }
 }
 
-// Advance the Anderson ring for every active load case before a step.
+// Advance the Anderson ring of every active load case before the x pass that
+// records a new residual: newest slot, fill count, and the Gram matrix shifted
+// by one age (row and column 0 are then set by k_aa_solve).
 __global__ void k_aa_advance(const RhsState* __restrict__ st, int* __restrict__ head, int* __restrict__ filled,
-                             int nrhs, int m1) {
+                             double* __restrict__ gram, int nrhs, int m1) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nrhs || !st[r].active) return;
   head[r] = (head[r] + 1) % m1;
   filled[r] = min(filled[r] + 1, m1);
+  double* A = gram + (long long)r * m1 * m1;
+  for (int i = m1 - 1; i >= 1; --i)
+    for (int j = m1 - 1; j >= 1; --j) A[i * m1 + j] = A[(i - 1) * m1 + (j - 1)];
 }
 
 }  // namespace lsk

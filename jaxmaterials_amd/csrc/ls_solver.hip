@@ -117,7 +117,7 @@ struct Tile {
   int E = 0;      // register elements per thread (LDS engine)
   int NT = 0;     // threads per block
   size_t lds = 0;
-  int reg = 0;    // 1: register two-stage kernel (k_ypass_r / k_xpass_r); 2: k_xpass_h
+  int reg = 0;    // 1: register two-stage y kernel (k_ypass_r); 2: three-line x kernel (k_xpass_h)
 };
 
 // Register two-stage kernels compiled for the axis lengths of the workloads
@@ -127,11 +127,6 @@ struct Tile {
   F(256, 16, 16, 16, 256) F(256, 16, 16, 8, 128) F(128, 16, 8, 32, 256) F(128, 16, 8, 48, 384)  \
   F(96, 12, 8, 32, 256) F(96, 12, 8, 64, 512) F(64, 8, 8, 32, 256) F(64, 8, 8, 64, 512)         \
   F(48, 6, 8, 32, 256) F(48, 6, 8, 64, 512) F(32, 8, 4, 64, 256) F(32, 8, 4, 128, 512)
-#define LS_REG_X_LIST(F)                                                                        \
-  F(256, 32, 8, 4, 192) F(256, 32, 8, 2, 128) F(128, 16, 8, 4, 192) F(128, 16, 8, 8, 384)      \
-  F(96, 12, 8, 4, 192) F(96, 12, 8, 8, 384) F(64, 8, 8, 4, 192) F(64, 8, 8, 8, 384)            \
-  F(48, 6, 8, 4, 192) F(48, 6, 8, 8, 384) F(32, 8, 4, 8, 192) F(32, 8, 4, 16, 384)
-
 // x pass with three-line transforms (k_xpass_h): (n, A, B, TM, NT)
 #define LS_REG_H_LIST(F)                                                                        \
   F(256, 32, 8, 4, 256) F(256, 16, 16, 4, 256) F(256, 32, 8, 4, 128) F(128, 16, 8, 8, 256)   \
@@ -144,13 +139,12 @@ struct RegCfg {
 };
 #define LS_AS_CFG(NV, AV, BV, LV, NTV) {NV, AV, BV, LV, NTV},
 const RegCfg kRegY[] = {LS_REG_Y_LIST(LS_AS_CFG)};
-const RegCfg kRegX[] = {LS_REG_X_LIST(LS_AS_CFG)};
 const RegCfg kRegH[] = {LS_REG_H_LIST(LS_AS_CFG)};
 #undef LS_AS_CFG
 
-// Register-kernel tiles of an axis; lines_per_unit = 1 (y columns) or 6 (x
-// modes); kind 2 = the three-line x pass (LDS: 3 TM (n + Np) complex).
-std::vector<Tile> reg_tiles(const RegCfg* list, size_t count, int n, int lines_per_unit, int kind = 1) {
+// Register-kernel tiles of an axis: kind 1 = y columns (LDS: T Np complex),
+// kind 2 = the three-line x pass (LDS: 3 TM (n + Np) complex).
+std::vector<Tile> reg_tiles(const RegCfg* list, size_t count, int n, int kind) {
   std::vector<Tile> out;
   for (size_t i = 0; i < count; ++i) {
     const RegCfg& c = list[i];
@@ -161,8 +155,7 @@ std::vector<Tile> reg_tiles(const RegCfg* list, size_t count, int n, int lines_p
     t.reg = kind;
     t.E = c.A;  // distinguishes the (A, B) splits of one length
     const int np = c.n + c.n / c.B + 1;
-    t.lds = kind == 2 ? (size_t)3 * c.lines * (c.n + np) * sizeof(float2)
-                      : (size_t)c.lines * lines_per_unit * np * sizeof(float2);
+    t.lds = kind == 2 ? (size_t)3 * c.lines * (c.n + np) * sizeof(float2) : (size_t)c.lines * np * sizeof(float2);
     out.push_back(t);
   }
   return out;
@@ -218,7 +211,7 @@ struct ls_solver {
   int absolute = 0;  // LS_ABSOLUTE_RESIDUAL flag of the current solve
   // Anderson
   float* aaG = nullptr;
-  float* aaR = nullptr;
+  float2* aaU = nullptr;
   double* aa_alpha = nullptr;
   double* aa_dots = nullptr;
   double* aa_gram = nullptr;
@@ -230,9 +223,11 @@ struct ls_solver {
   double gtol[2] = {0.0, 0.0};
   int gsteps = 0, gabs = -1;
   // timing of the last solve (LS_TIMING set): HIP events on the solver stream
-  static constexpr int kKinds = 6;  // z (plain), y, x, other, z Anderson residual, z Anderson mix
+  static constexpr int kKinds = 6;  // z (plain), y, x, other, z Anderson, Anderson Gram row
   double t_ms[kKinds] = {0, 0, 0, 0, 0, 0};
   int t_cnt[kKinds] = {0, 0, 0, 0, 0, 0};
+  int nblk_g = 0;           // blocks per load case of the Gram-row kernel
+  long long chunk_g = 0;    // complex entries per block
   bool timing = false;
   std::vector<hipEvent_t> tev;
   std::vector<int> tkind;
@@ -330,16 +325,22 @@ void mark(ls_solver* s, int kind) {
 }
 
 // ---- kernel launch dispatch on E ------------------------------------------
-template <int MODE>
-void zline(ls_solver* s, const float* lam, const float* mu, float* eps) {
+AAParams aa_params(const ls_solver* s, int record) {
   AAParams aa;
   aa.G = s->aaG;
-  aa.Rh = s->aaR;
+  aa.U = s->aaU;
+  aa.u_per_rhs = s->dl.per_rhs;
   aa.alpha = s->aa_alpha;
-  aa.dots = s->aa_dots;
   aa.head = s->aa_head;
   aa.filled = s->aa_filled;
   aa.m1 = s->m1;
+  aa.record = (record && s->depth > 0) ? 1 : 0;
+  return aa;
+}
+
+template <int MODE>
+void zline(ls_solver* s, const float* lam, const float* mu, float* eps) {
+  const AAParams aa = aa_params(s, 0);
   const unsigned grid = (unsigned)s->nblk_z * s->nrhs;
   const size_t lds = s->tz.lds;
 #define LS_Z(EV, NTV)                                                                                           \
@@ -347,7 +348,7 @@ void zline(ls_solver* s, const float* lam, const float* mu, float* eps) {
                                                         s->nblk_z, aa)
   LS_DISPATCH32(s->tz, LS_Z)
 #undef LS_Z
-  mark(s, MODE == Z_AAINV ? 4 : (MODE == Z_AAMIX ? 5 : 0));
+  mark(s, MODE == Z_AA ? 4 : 0);
 }
 
 void ypass(ls_solver* s, bool inv) {
@@ -380,15 +381,22 @@ void ypass(ls_solver* s, bool inv) {
   mark(s, 1);
 }
 
-void xpass(ls_solver* s, int do_gamma, bool with_base = true) {
+// x pass over all active load cases; record: store the residual's spectral
+// form and its Gram row for Anderson (solvers with depth > 0).
+void xpass(ls_solver* s, int do_gamma, bool with_base = true, bool record = false) {
+  const AAParams aa = aa_params(s, record ? 1 : 0);
   const unsigned grid = (unsigned)s->nblk_x * s->nrhs;
   const size_t lds = s->tx.lds;
   const float2* db = with_base ? s->dbase : nullptr;
   if (s->tx.reg == 2) {
 #define LS_XH(NV, AV, BV, LV, NTV)                                                                        \
   else if (s->nx == NV && s->tx.lines == LV && s->tx.NT == NTV && s->tx.E == AV) {                       \
-    k_xpass_h<AV, BV, LV, NTV><<<grid, NTV, lds, s->stream>>>(s->gd, s->d_rhs, s->d_st, s->spec, db, s->partials, \
-                                                              s->dc, s->nblk_x, s->dl.per_rhs, do_gamma);   \
+    if (aa.record)                                                                                        \
+      k_xpass_h<AV, BV, LV, NTV, true><<<grid, NTV, lds, s->stream>>>(                                     \
+          s->gd, s->d_rhs, s->d_st, s->spec, db, s->partials, s->dc, s->nblk_x, s->dl.per_rhs, do_gamma, aa); \
+    else                                                                                                  \
+      k_xpass_h<AV, BV, LV, NTV, false><<<grid, NTV, lds, s->stream>>>(                                    \
+          s->gd, s->d_rhs, s->d_st, s->spec, db, s->partials, s->dc, s->nblk_x, s->dl.per_rhs, do_gamma, aa); \
   }
     if (false) {
     }
@@ -398,24 +406,14 @@ void xpass(ls_solver* s, int do_gamma, bool with_base = true) {
     mark(s, 2);
     return;
   }
-  if (s->tx.reg) {
-#define LS_XR(NV, AV, BV, LV, NTV)                                                                        \
-  else if (s->nx == NV && s->tx.lines == LV && s->tx.NT == NTV && s->tx.E == AV) {                       \
-    k_xpass_r<AV, BV, LV, NTV><<<grid, NTV, lds, s->stream>>>(s->gd, s->d_rhs, s->d_st, s->spec, db, s->partials, \
-                                                              s->dc, s->nblk_x, s->dl.per_rhs, do_gamma);   \
-  }
-    if (false) {
-    }
-    LS_REG_X_LIST(LS_XR)
-    else fprintf(stderr, "liblippmannschwinger: no register x kernel for nx=%d TM=%d\n", s->nx, s->tx.lines);
-#undef LS_XR
-    mark(s, 2);
-    return;
-  }
   const FastDiv fTm = make_fastdiv(s->tx.lines), fcols = make_fastdiv(6 * s->tx.lines);
 #define LS_X(EV, NTV)                                                                                      \
-  k_xpass<EV, NTV><<<grid, NTV, lds, s->stream>>>(s->gd, s->d_rhs, s->d_st, s->spec, db, s->partials, s->dc, \
-                                                  fTm, fcols, s->nblk_x, s->dl, do_gamma)
+  if (aa.record)                                                                                            \
+    k_xpass<EV, NTV, true><<<grid, NTV, lds, s->stream>>>(s->gd, s->d_rhs, s->d_st, s->spec, db, s->partials, s->dc, \
+                                                          fTm, fcols, s->nblk_x, s->dl, do_gamma, aa);         \
+  else                                                                                                      \
+    k_xpass<EV, NTV, false><<<grid, NTV, lds, s->stream>>>(s->gd, s->d_rhs, s->d_st, s->spec, db, s->partials,     \
+                                                           s->dc, fTm, fcols, s->nblk_x, s->dl, do_gamma, aa)
   LS_DISPATCH32(s->tx, LS_X)
 #undef LS_X
   mark(s, 2);
@@ -503,7 +501,7 @@ void free_solver(ls_solver* s) {
     (void)hipFree(a.d_ph);
   }
   void* dev[] = {s->spec,    s->spec64,  s->base,    s->dbase,    s->partials, s->dc,       s->d_rhs,
-                 s->d_st,    s->d_sel,   s->part6,   s->d_mean6,  s->d_mm,     s->aaG,      s->aaR,
+                 s->d_st,    s->d_sel,   s->part6,   s->d_mean6,  s->d_mm,     s->aaG,      s->aaU,
                  s->aa_alpha, s->aa_dots, s->aa_gram, s->aa_head, s->aa_filled};
   for (void* p : dev) (void)hipFree(p);
   (void)hipHostFree(s->h_st);
@@ -522,8 +520,12 @@ void apply_tiles(ls_solver* s) {
   s->nblk_y = s->nx * s->ntiles_y;
   s->nblk_x = (int)(((long long)s->ny * s->P + s->tx.lines - 1) / s->tx.lines);
   s->dl.f_Tm = make_fastdiv(s->tx.lines);
+  s->dl.hA = s->tx.reg == 2 ? s->tx.E : 0;
+  s->dl.f_A = make_fastdiv(s->dl.hA > 0 ? s->dl.hA : 1);
   s->dl.nx = s->nx;
-  s->dl.per_rhs = ((long long)s->ny * s->P + kMaxTm) * s->nx * 3;
+  // a multiple of 4 complex entries, so every slot of the Anderson y ring
+  // starts 32-byte aligned
+  s->dl.per_rhs = ((((long long)s->ny * s->P + kMaxTm) * s->nx * 3) + 3) / 4 * 4;
 }
 
 int create_impl(ls_solver* s, const int* voxels, const float* extents, int nrhs, int nmat, int depth,
@@ -582,13 +584,9 @@ int create_impl(ls_solver* s, const int* voxels, const float* extents, int nrhs,
   }
   if (env_int("LS_REG_FFT", 1)) {
     const std::vector<Tile> ry = reg_tiles(kRegY, sizeof(kRegY) / sizeof(kRegY[0]), s->ny, 1);
-    const std::vector<Tile> rx = reg_tiles(kRegX, sizeof(kRegX) / sizeof(kRegX[0]), s->nx, 6);
-    const std::vector<Tile> rh = reg_tiles(kRegH, sizeof(kRegH) / sizeof(kRegH[0]), s->nx, 6, 2);
+    const std::vector<Tile> rh = reg_tiles(kRegH, sizeof(kRegH) / sizeof(kRegH[0]), s->nx, 2);
     if (!ry.empty()) s->ty = ry[0];
-    if (!rh.empty())
-      s->tx = rh[0];
-    else if (!rx.empty())
-      s->tx = rx[0];
+    if (!rh.empty()) s->tx = rh[0];
   }
   if (s->tx.lines > kMaxTm) {
     fprintf(stderr, "liblippmannschwinger: x tile of %d modes exceeds %d\n", s->tx.lines, kMaxTm);
@@ -640,8 +638,12 @@ int create_impl(ls_solver* s, const int* voxels, const float* extents, int nrhs,
   LS_HIP(hipHostMalloc((void**)&s->h_mm, (size_t)nmat * s->nblk_mm * 4 * sizeof(float), hipHostMallocDefault));
   if (depth > 0) {
     const size_t hist = (size_t)nrhs * s->m1 * 6 * s->N;
-    if (dalloc(&s->aaG, hist) || dalloc(&s->aaR, hist) || dalloc(&s->aa_alpha, (size_t)nrhs * s->m1) ||
-        dalloc(&s->aa_dots, (size_t)nrhs * s->nx * s->ny * s->m1) || dalloc(&s->aa_gram, (size_t)nrhs * s->m1 * s->m1) ||
+    // y ring in the d_base layout; Gram-row partials per k_aa_gram block
+    s->nblk_g = (int)std::min<long long>(512, (s->dl.per_rhs + 16383) / 16384);
+    s->chunk_g = ((s->dl.per_rhs + s->nblk_g - 1) / s->nblk_g + 1) / 2 * 2;
+    if (dalloc(&s->aaG, hist) || dalloc(&s->aaU, (size_t)nrhs * s->m1 * s->dl.per_rhs) ||
+        dalloc(&s->aa_alpha, (size_t)nrhs * s->m1) || dalloc(&s->aa_dots, (size_t)nrhs * s->nblk_g * s->m1) ||
+        dalloc(&s->aa_gram, (size_t)nrhs * s->m1 * s->m1) ||
         dalloc(&s->aa_head, nrhs) || dalloc(&s->aa_filled, nrhs))
       return -1;
   }
@@ -657,6 +659,8 @@ int reset_aa(ls_solver* s) {
   LS_HIP(hipMemcpyAsync(s->aa_gram, gram.data(), gram.size() * sizeof(double), hipMemcpyHostToDevice, s->stream));
   LS_HIP(hipMemcpyAsync(s->aa_head, head.data(), head.size() * sizeof(int), hipMemcpyHostToDevice, s->stream));
   LS_HIP(hipMemcpyAsync(s->aa_filled, filled.data(), filled.size() * sizeof(int), hipMemcpyHostToDevice, s->stream));
+  // entries of the y ring outside the grid's modes are never written: keep them zero
+  LS_HIP(hipMemsetAsync(s->aaU, 0, (size_t)s->nrhs * s->m1 * s->dl.per_rhs * sizeof(float2), s->stream));
   LS_HIP(hipStreamSynchronize(s->stream));
   return 0;
 }
@@ -666,11 +670,31 @@ void init_fields(ls_solver* s, const int* d_sel, float* delta, const float* src)
   k_init_fields<<<grid, kThreads, 0, s->stream>>>(s->d_rhs, d_sel, s->N, s->base, delta, src);
 }
 
-// One residual evaluation after a z-line forward pass: y fwd, x (residual + Gamma0), check.
+// Residual evaluation of the current iterate from the spectrum after the
+// forward z and y passes: x pass (residual, Gamma0) and stop test.  With
+// Anderson the x pass records the residual and its Gram row (in a new ring
+// slot if `advance`, else replacing the newest entry, e.g. after a rebase) and
+// the weights of the next z pass are solved.
+void eval_x(ls_solver* s, double rtol, double atol, int mode, bool advance) {
+  if (s->depth > 0 && advance)
+    k_aa_advance<<<(s->nrhs + 63) / 64, 64, 0, s->stream>>>(s->d_st, s->aa_head, s->aa_filled, s->aa_gram, s->nrhs,
+                                                           s->m1);
+  xpass(s, 1, true, s->depth > 0);
+  check(s, rtol, atol, mode);
+  if (s->depth > 0) {
+    k_aa_gram<<<dim3((unsigned)s->nblk_g, (unsigned)s->nrhs), kThreads, 0, s->stream>>>(s->d_st, aa_params(s, 1),
+                                                                                      s->aa_dots, s->chunk_g);
+    mark(s, 5);
+    k_aa_solve<<<s->nrhs, kThreads, 0, s->stream>>>(s->d_st, s->aa_dots, s->nblk_g, s->m1, s->aa_gram,
+                                                    s->aa_alpha, s->aa_head, s->aa_filled);
+    mark(s, 3);
+  }
+}
+
+// One residual evaluation after a z-line forward pass: y fwd, then eval_x.
 void evaluate(ls_solver* s, double rtol, double atol, int mode) {
   ypass(s, false);
-  xpass(s, 1);
-  check(s, rtol, atol, mode);
+  eval_x(s, rtol, atol, mode, true);
 }
 
 // One iteration for all active load cases (the spectrum holds Gamma0(d) after
@@ -678,16 +702,10 @@ void evaluate(ls_solver* s, double rtol, double atol, int mode) {
 void iterate(ls_solver* s, const float* lam, const float* mu, float* eps, double rtol, double atol,
              bool evaluate_after) {
   ypass(s, true);
-  if (s->depth == 0) {
+  if (s->depth == 0)
     zline<Z_UPD>(s, lam, mu, eps);
-  } else {
-    k_aa_advance<<<(s->nrhs + 63) / 64, 64, 0, s->stream>>>(s->d_st, s->aa_head, s->aa_filled, s->nrhs, s->m1);
-    zline<Z_AAINV>(s, lam, mu, eps);
-    k_aa_solve<<<s->nrhs, kThreads, 0, s->stream>>>(s->d_st, s->aa_dots, s->nblk_z, s->m1, s->aa_gram,
-                                                    s->aa_alpha, s->aa_head, s->aa_filled);
-    mark(s, 3);
-    zline<Z_AAMIX>(s, lam, mu, eps);
-  }
+  else
+    zline<Z_AA>(s, lam, mu, eps);
   if (evaluate_after) evaluate(s, rtol, atol, 0);
 }
 
@@ -731,8 +749,7 @@ int rebase(ls_solver* s, const float* lam, const float* mu, const float* tau, fl
   k_rebase<<<grid, kThreads, 0, s->stream>>>(s->d_st, s->N, s->base, eps, s->aaG, s->aa_filled, s->m1);
   base_eval_active(s, lam, mu, tau);
   if (zero_spec(s, active_mask(s))) return -1;
-  xpass(s, 1);
-  check(s, rtol, atol, 2);
+  eval_x(s, rtol, atol, 2, false);
   ++s->n_rebase;
   return 0;
 }
@@ -836,8 +853,7 @@ int solve_impl(ls_solver* s, const float* d_lambda, const float* d_mu, const int
     init_fields(s, nullptr, d_eps, d_eps);
     base_eval_active(s, d_lambda, d_mu, d_tau);
     if (zero_spec(s, all)) return -1;
-    xpass(s, 1);
-    check(s, drtol, datol, 0);
+    eval_x(s, drtol, datol, 0, true);
     if (sync_state(s)) return -1;
     // 4. fall back to the uniform start where the warm start is worse
     std::vector<int> restart(s->nrhs), resume(s->nrhs);
@@ -862,8 +878,7 @@ int solve_impl(ls_solver* s, const float* d_lambda, const float* d_mu, const int
       init_fields(s, s->d_sel, d_eps, nullptr);
       base_eval_active(s, d_lambda, d_mu, d_tau);
       if (zero_spec(s, restart)) return -1;
-      xpass(s, 1);
-      check(s, drtol, datol, 0);
+      eval_x(s, drtol, datol, 0, false);
       if (sync_state(s)) return -1;
       for (int r = 0; r < s->nrhs; ++r)
         if (resume[r]) s->h_st[r].active = 1;
@@ -873,8 +888,7 @@ int solve_impl(ls_solver* s, const float* d_lambda, const float* d_mu, const int
     init_fields(s, nullptr, d_eps, nullptr);
     base_eval_active(s, d_lambda, d_mu, d_tau);
     if (zero_spec(s, all)) return -1;
-    xpass(s, 1);
-    check(s, drtol, datol, 0);
+    eval_x(s, drtol, datol, 0, true);
   }
   LS_HIP(hipGetLastError());
 
@@ -1084,8 +1098,7 @@ int autotune_impl(ls_solver* s, const float* lam, const float* mu, float* eps, i
   std::vector<Tile> cy = tile_candidates(s->ax[1], wants(wy), 1, (size_t)s->ny * sizeof(float2), 6 * s->P);
   std::vector<Tile> cx = tile_candidates(s->ax[0], wants(wx), 6, (size_t)6 * s->nx * sizeof(float2), kMaxTm);
   for (const Tile& t : reg_tiles(kRegY, sizeof(kRegY) / sizeof(kRegY[0]), s->ny, 1)) cy.push_back(t);
-  for (const Tile& t : reg_tiles(kRegX, sizeof(kRegX) / sizeof(kRegX[0]), s->nx, 6)) cx.push_back(t);
-  for (const Tile& t : reg_tiles(kRegH, sizeof(kRegH) / sizeof(kRegH[0]), s->nx, 6, 2)) cx.push_back(t);
+  for (const Tile& t : reg_tiles(kRegH, sizeof(kRegH) / sizeof(kRegH[0]), s->nx, 2)) cx.push_back(t);
   if (sweep("z", s->tz, tile_candidates(s->ax[2], wants(wz), 3, (size_t)3 * lsz * sizeof(float2), 1 << 20),
             [&] { zline<Z_UPD>(s, lam, mu, eps); }) ||
       sweep("y", s->ty, cy, [&] { ypass(s, true); }) || sweep("x", s->tx, cx, [&] { xpass(s, 1); }))

@@ -137,14 +137,15 @@ class LSSolver:
             raise RuntimeError("ls_solver_autotune failed (see stderr)")
         return buf.value.decode()
 
-    KINDS = ("z", "y", "x", "other", "z_aa_residual", "z_aa_mix")
+    KINDS = ("z", "y", "x", "other", "z_aa", "aa_gram")
 
     def last_timing(self):
         """Kernel milliseconds and launch counts of the last solve by pass kind
         (``LSSolver.KINDS``), recorded when the environment variable LS_TIMING
         was set; also the number of base re-evaluations."""
-        ms = np.zeros(6, dtype=np.float64)
-        cnt = np.zeros(6, dtype=np.int32)
+        nk = len(self.KINDS)
+        ms = np.zeros(nk, dtype=np.float64)
+        cnt = np.zeros(nk, dtype=np.int32)
         nreb = self.lib.ls_solver_last_timing(self._h, ms.ctypes.data_as(ctypes.c_void_p),
-                                              cnt.ctypes.data_as(ctypes.c_void_p), 6)
+                                              cnt.ctypes.data_as(ctypes.c_void_p), nk)
         return dict(zip(self.KINDS, ms.tolist())), dict(zip(self.KINDS, cnt.tolist())), int(nreb)
