@@ -219,10 +219,11 @@ def test_fp32_noise_floor_crossply(gpu, n):
     assert err <= 1e-5
 
 
-# Axis lengths with a register two-stage FFT kernel in the y and x passes
-# (256 = 16x16 / 32x8, 128, 96, 64, 48, 32); each grid puts one of them on x
-# and another on y with a short z axis so the FP64 oracle stays cheap.
-LONG_AXES = [(256, 96, 8), (96, 256, 8), (128, 48, 10), (48, 128, 10), (32, 64, 12), (64, 32, 12)]
+# Axis lengths with register two-stage FFT kernels in the y and x passes
+# (256, 128, 96, 64, 48, 32), and the same lengths on z; each grid keeps the
+# other axes short so the FP64 oracle stays cheap.
+LONG_AXES = [(256, 96, 8), (96, 256, 8), (128, 48, 10), (48, 128, 10), (32, 64, 12), (64, 32, 12),
+             (12, 10, 256), (10, 12, 128), (16, 12, 96), (10, 14, 48)]
 
 
 @pytest.mark.parametrize("shape", LONG_AXES, ids=lambda s: "x".join(map(str, s)))
