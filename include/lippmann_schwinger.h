@@ -81,9 +81,10 @@ int ls_solver_solve(ls_solver *s, const float *d_lambda, const float *d_mu,
                     double rel_ref, int flags, int verbose, int *iters,
                     double *residual, double *sigma_mean);
 
-/* Autotuner: time every candidate tile (lines per block, register elements
- * and threads) of the z-line, y and x passes on the solver's own buffers and
- * keep the fastest for later solves.  d_lambda, d_mu [nmat][N] and d_eps
+/* Autotuner: time every candidate tile of the z-line, y and x passes (lines
+ * per block, register elements and threads of the LDS FFT engine, and the
+ * register-FFT kernels compiled for the axis length) on the solver's own
+ * buffers and keep the fastest for later solves.  d_lambda, d_mu [nmat][N] and d_eps
  * [nrhs][6][N] are device buffers (d_eps is overwritten); reps launches per
  * candidate.  report (may be NULL) receives the chosen tiles and times.
  * Returns 0, or -1 on error. */
@@ -93,9 +94,9 @@ int ls_solver_autotune(ls_solver *s, const float *d_lambda, const float *d_mu,
 /* Kernel time of the last solve (recorded with HIP events on the solver's
  * stream when the environment variable LS_TIMING is set), ms and launch
  * counts per pass kind: [0] z-line passes (plain), [1] y passes, [2] x passes,
- * [3] everything else, [4] z-line Anderson residual passes, [5] z-line
- * Anderson mixing passes.  Fills the first nkinds kinds; returns the number
- * of base re-evaluations of the last solve. */
+ * [3] everything else, [4] z-line Anderson passes (update, mixing, Hooke,
+ * forward transform), [5] Anderson Gram-row reductions.  Fills the first
+ * nkinds kinds; returns the number of base re-evaluations of the last solve. */
 int ls_solver_last_timing(ls_solver *s, double *ms, int *count, int nkinds);
 
 void ls_solver_destroy(ls_solver *s);

@@ -80,8 +80,11 @@ def lame_fields(p, device="cuda"):
 
 def _batch_size(n, depth, budget_bytes):
     """Microstructures per solver call within a device-memory budget
-    (about (12 m + 28) fields of N floats per load case, m = depth)."""
-    per_rhs = (12 * (depth + 1) + 28) * n**3 * 4
+    (about 9 (m + 1) + 21 fields of N floats per load case, m = depth: base,
+    correction, half spectrum, base divergence, and the Anderson rings of
+    g (6 N per slot) and of the spectral residual form (3 N per slot);
+    a margin of 3 fields is added)."""
+    per_rhs = (9 * (depth + 1) + 24) * n**3 * 4
     return max(1, int(budget_bytes // (6 * per_rhs)))
 
 
