@@ -101,6 +101,20 @@ int ls_solver_last_timing(ls_solver *s, double *ms, int *count, int nkinds);
 
 void ls_solver_destroy(ls_solver *s);
 
+/* Real-space relative divergence of a stress field (device pointers):
+ * ||D^- . sigma|| / (sqrt(N) ||<sigma>||_V) with the backward derivative of
+ * the rotated staggered grid, the reference's `relative_divergence`
+ * (src/jaxmaterials/solver/lippmann_schwinger.py:103-115) over the stencil of
+ * src/jaxmaterials/solver/derivatives.py:8-96 and
+ * cuda/src/derivatives.cu:9-124.  d_sigma [6][N]; d_div [3][N] receives
+ * D^- . sigma (may be NULL); *rel the relative divergence (sums in FP64).
+ * stream = hipStream_t (NULL: the default stream); synchronous.  Returns 0,
+ * or -1 on error.  An independent check of the solver's Fourier residual,
+ * equal to it by Parseval. */
+int ls_relative_divergence(const int *voxels, const float *extents,
+                           const float *d_sigma, float *d_div, double *rel,
+                           void *stream);
+
 /* Seeded cross-ply fibre microstructure (random sequential adsorption) and
  * its rasterisation into Lame fields on the GPU.
  *   nlayers layers along z; layer k has fibres along x (k odd) or y (k even);

@@ -48,6 +48,10 @@ SIGNATURES = {
         [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_ulonglong,
          ctypes.c_void_p, ctypes.c_int],
     ),
+    "ls_relative_divergence": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+    ),
     # learned warm start (include/ls_warmstart.h)
     "ls_ws_conv3d": (
         ctypes.c_int,

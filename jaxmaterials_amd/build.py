@@ -16,7 +16,7 @@ LIB_NAME = "liblippmannschwinger.so"
 LIB_PATH = os.path.join(HERE, LIB_NAME)
 
 # every HIP source of the library; each compiles to its own object in parallel
-SOURCES = ["ls_solver.hip", "microstructure.hip", "conv3d.hip"]
+SOURCES = ["ls_solver.hip", "microstructure.hip", "conv3d.hip", "derivatives.hip"]
 
 HIPCC_FLAGS = [
     "-O3",
