@@ -100,7 +100,10 @@ namespace lsk {
 //
 // LDS: h and then g lines [x][3 TM] in the first 3 TM N slots, the stage-1
 // transforms in RegLayout after them; per-mode coefficients in a small table.
-template <int A, int B, int TM, int NT, bool REC>
+// INPL: the transforms overwrite the h / g lines in place (each thread holds
+// at most one line segment of stages 1 and 3 in registers across a barrier),
+// which halves the LDS of a block.
+template <int A, int B, int TM, int NT, bool REC, bool INPL>
 __global__ void __launch_bounds__(NT) k_xpass_h(GridDev g, const RhsDesc* __restrict__ rhs,
                                                 const RhsState* __restrict__ st, float2* __restrict__ spec,
                                                 const float2* __restrict__ dbase, double* __restrict__ partials,
@@ -109,9 +112,10 @@ __global__ void __launch_bounds__(NT) k_xpass_h(GridDev g, const RhsDesc* __rest
   constexpr int N = A * B;
   constexpr int LH = 3 * TM;  // h / g lines
   using L = RegLayout<N, B>;
+  static_assert(!INPL || B * LH <= NT, "in-place stages 1 and 3 need one item per thread");
   extern __shared__ float2 lds[];
-  float2* H = lds;           // [x][LH]
-  float2* W = lds + LH * N;  // RegLayout lines 3m + i
+  float2* H = lds;                       // [x][LH]
+  float2* W = INPL ? lds : lds + LH * N;  // RegLayout lines 3m + i
   __shared__ double red[NT / 64];
   // per mode: alpha, beta, gamma (xi~ = (alpha s_x, beta c_x, gamma c_x)),
   // c_y c_z, s_y c_z, c_y s_z and the Hermitian weight
@@ -186,16 +190,34 @@ __global__ void __launch_bounds__(NT) k_xpass_h(GridDev g, const RhsDesc* __rest
   __syncthreads();
 
   // stage 1 (forward): item (t, line), A-point DFTs over x = t + B j
-  for (int it = threadIdx.x; it < B * LH; it += NT) {
+  if constexpr (INPL) {
+    const int it = threadIdx.x;
     const int t = it / LH, l = it - t * LH;
     float2 v[A];
+    if (it < B * LH) {
 #pragma unroll
-    for (int j = 0; j < A; ++j) v[j] = H[(t + B * j) * LH + l];
-    dft_reg<A, false>(v);
+      for (int j = 0; j < A; ++j) v[j] = H[(t + B * j) * LH + l];
+      dft_reg<A, false>(v);
 #pragma unroll
-    for (int k1 = 1; k1 < A; ++k1) v[k1] = cmul(v[k1], tw[t * k1]);
+      for (int k1 = 1; k1 < A; ++k1) v[k1] = cmul(v[k1], tw[t * k1]);
+    }
+    __syncthreads();
+    if (it < B * LH) {
 #pragma unroll
-    for (int k1 = 0; k1 < A; ++k1) W[L::at(l, t + B * k1)] = v[k1];
+      for (int k1 = 0; k1 < A; ++k1) W[L::at(l, t + B * k1)] = v[k1];
+    }
+  } else {
+    for (int it = threadIdx.x; it < B * LH; it += NT) {
+      const int t = it / LH, l = it - t * LH;
+      float2 v[A];
+#pragma unroll
+      for (int j = 0; j < A; ++j) v[j] = H[(t + B * j) * LH + l];
+      dft_reg<A, false>(v);
+#pragma unroll
+      for (int k1 = 1; k1 < A; ++k1) v[k1] = cmul(v[k1], tw[t * k1]);
+#pragma unroll
+      for (int k1 = 0; k1 < A; ++k1) W[L::at(l, t + B * k1)] = v[k1];
+    }
   }
   __syncthreads();
 
@@ -273,14 +295,30 @@ __global__ void __launch_bounds__(NT) k_xpass_h(GridDev g, const RhsDesc* __rest
   __syncthreads();
 
   // stage 3 (inverse): item (t, line): A-point inverse DFTs, g(x = t + B j) into H
-  for (int it = threadIdx.x; it < B * LH; it += NT) {
+  if constexpr (INPL) {
+    const int it = threadIdx.x;
     const int t = it / LH, l = it - t * LH;
     float2 v[A];
+    if (it < B * LH) {
 #pragma unroll
-    for (int k1 = 0; k1 < A; ++k1) v[k1] = W[L::at(l, t + B * k1)];
-    dft_reg<A, true>(v);
+      for (int k1 = 0; k1 < A; ++k1) v[k1] = W[L::at(l, t + B * k1)];
+      dft_reg<A, true>(v);
+    }
+    __syncthreads();
+    if (it < B * LH) {
 #pragma unroll
-    for (int j = 0; j < A; ++j) H[(t + B * j) * LH + l] = v[j];
+      for (int j = 0; j < A; ++j) H[(t + B * j) * LH + l] = v[j];
+    }
+  } else {
+    for (int it = threadIdx.x; it < B * LH; it += NT) {
+      const int t = it / LH, l = it - t * LH;
+      float2 v[A];
+#pragma unroll
+      for (int k1 = 0; k1 < A; ++k1) v[k1] = W[L::at(l, t + B * k1)];
+      dft_reg<A, true>(v);
+#pragma unroll
+      for (int j = 0; j < A; ++j) H[(t + B * j) * LH + l] = v[j];
+    }
   }
   __syncthreads();
 

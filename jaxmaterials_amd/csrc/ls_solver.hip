@@ -118,6 +118,7 @@ struct Tile {
   int NT = 0;     // threads per block
   size_t lds = 0;
   int reg = 0;    // 1: register two-stage y kernel (k_ypass_r); 2: three-line x kernel (k_xpass_h)
+  int inpl = 0;   // k_xpass_h: transforms in place (half the LDS)
 };
 
 // Register two-stage kernels compiled for the axis lengths of the workloads
@@ -127,20 +128,23 @@ struct Tile {
   F(256, 16, 16, 16, 256) F(256, 16, 16, 8, 128) F(128, 16, 8, 32, 256) F(128, 16, 8, 48, 384)  \
   F(96, 12, 8, 32, 256) F(96, 12, 8, 64, 512) F(64, 8, 8, 32, 256) F(64, 8, 8, 64, 512)         \
   F(48, 6, 8, 32, 256) F(48, 6, 8, 64, 512) F(32, 8, 4, 64, 256) F(32, 8, 4, 128, 512)
-// x pass with three-line transforms (k_xpass_h): (n, A, B, TM, NT)
-#define LS_REG_H_LIST(F)                                                                        \
-  F(256, 32, 8, 4, 256) F(256, 16, 16, 4, 256) F(256, 32, 8, 4, 128) F(128, 16, 8, 8, 256)   \
-  F(128, 16, 8, 4, 256) F(96, 12, 8, 8, 256) F(96, 12, 8, 4, 128) F(64, 8, 8, 8, 256)          \
-  F(64, 8, 8, 16, 512) F(48, 6, 8, 16, 256) F(48, 6, 8, 8, 256) F(32, 8, 4, 16, 256)          \
-  F(32, 8, 4, 32, 512)
+// x pass with three-line transforms (k_xpass_h): (n, A, B, TM, NT, in place)
+#define LS_REG_H_LIST(F)                                                                                     \
+  F(256, 32, 8, 8, 256, 1) F(256, 32, 8, 4, 256, 0) F(256, 16, 16, 4, 256, 0) F(256, 32, 8, 4, 128, 0)      \
+  F(256, 32, 8, 4, 128, 1) F(128, 16, 8, 8, 256, 0) F(128, 16, 8, 4, 256, 0) F(128, 16, 8, 8, 256, 1)        \
+  F(96, 12, 8, 8, 256, 0) F(96, 12, 8, 4, 128, 0) F(96, 12, 8, 8, 256, 1) F(64, 8, 8, 8, 256, 0)             \
+  F(64, 8, 8, 16, 512, 0) F(64, 8, 8, 16, 512, 1) F(48, 6, 8, 16, 256, 0) F(48, 6, 8, 8, 256, 0)             \
+  F(32, 8, 4, 16, 256, 0) F(32, 8, 4, 32, 512, 0)
 
 struct RegCfg {
-  int n, A, B, lines, NT;
+  int n, A, B, lines, NT, inpl;
 };
-#define LS_AS_CFG(NV, AV, BV, LV, NTV) {NV, AV, BV, LV, NTV},
+#define LS_AS_CFG(NV, AV, BV, LV, NTV) {NV, AV, BV, LV, NTV, 0},
+#define LS_AS_CFG6(NV, AV, BV, LV, NTV, IP) {NV, AV, BV, LV, NTV, IP},
 const RegCfg kRegY[] = {LS_REG_Y_LIST(LS_AS_CFG)};
-const RegCfg kRegH[] = {LS_REG_H_LIST(LS_AS_CFG)};
+const RegCfg kRegH[] = {LS_REG_H_LIST(LS_AS_CFG6)};
 #undef LS_AS_CFG
+#undef LS_AS_CFG6
 
 // Register-kernel tiles of an axis: kind 1 = y columns (LDS: T Np complex),
 // kind 2 = the three-line x pass (LDS: 3 TM (n + Np) complex).
@@ -154,8 +158,10 @@ std::vector<Tile> reg_tiles(const RegCfg* list, size_t count, int n, int kind) {
     t.NT = c.NT;
     t.reg = kind;
     t.E = c.A;  // distinguishes the (A, B) splits of one length
+    t.inpl = c.inpl;
     const int np = c.n + c.n / c.B + 1;
-    t.lds = kind == 2 ? (size_t)3 * c.lines * (c.n + np) * sizeof(float2) : (size_t)c.lines * np * sizeof(float2);
+    t.lds = kind == 2 ? (size_t)3 * c.lines * (c.inpl ? np : c.n + np) * sizeof(float2)
+                      : (size_t)c.lines * np * sizeof(float2);
     out.push_back(t);
   }
   return out;
@@ -226,6 +232,7 @@ struct ls_solver {
   static constexpr int kKinds = 6;  // z (plain), y, x, other, z Anderson, Anderson Gram row
   double t_ms[kKinds] = {0, 0, 0, 0, 0, 0};
   int t_cnt[kKinds] = {0, 0, 0, 0, 0, 0};
+  bool launch_failed = false;  // a pass found no kernel for its tile (set by ypass / xpass)
   int nblk_g = 0;           // blocks per load case of the Gram-row kernel
   long long chunk_g = 0;    // complex entries per block
   bool timing = false;
@@ -365,7 +372,10 @@ void ypass(ls_solver* s, bool inv) {
     if (false) {
     }
     LS_REG_Y_LIST(LS_YR)
-    else fprintf(stderr, "liblippmannschwinger: no register y kernel for ny=%d T=%d\n", s->ny, s->ty.lines);
+    else {
+      fprintf(stderr, "liblippmannschwinger: no register y kernel for ny=%d T=%d\n", s->ny, s->ty.lines);
+      s->launch_failed = true;
+    }
 #undef LS_YR
     mark(s, 1);
     return;
@@ -389,19 +399,22 @@ void xpass(ls_solver* s, int do_gamma, bool with_base = true, bool record = fals
   const size_t lds = s->tx.lds;
   const float2* db = with_base ? s->dbase : nullptr;
   if (s->tx.reg == 2) {
-#define LS_XH(NV, AV, BV, LV, NTV)                                                                        \
-  else if (s->nx == NV && s->tx.lines == LV && s->tx.NT == NTV && s->tx.E == AV) {                       \
+#define LS_XH(NV, AV, BV, LV, NTV, IP)                                                                    \
+  else if (s->nx == NV && s->tx.lines == LV && s->tx.NT == NTV && s->tx.E == AV && s->tx.inpl == IP) {   \
     if (aa.record)                                                                                        \
-      k_xpass_h<AV, BV, LV, NTV, true><<<grid, NTV, lds, s->stream>>>(                                     \
+      k_xpass_h<AV, BV, LV, NTV, true, IP><<<grid, NTV, lds, s->stream>>>(                                 \
           s->gd, s->d_rhs, s->d_st, s->spec, db, s->partials, s->dc, s->nblk_x, s->dl.per_rhs, do_gamma, aa); \
     else                                                                                                  \
-      k_xpass_h<AV, BV, LV, NTV, false><<<grid, NTV, lds, s->stream>>>(                                    \
+      k_xpass_h<AV, BV, LV, NTV, false, IP><<<grid, NTV, lds, s->stream>>>(                                \
           s->gd, s->d_rhs, s->d_st, s->spec, db, s->partials, s->dc, s->nblk_x, s->dl.per_rhs, do_gamma, aa); \
   }
     if (false) {
     }
     LS_REG_H_LIST(LS_XH)
-    else fprintf(stderr, "liblippmannschwinger: no three-line x kernel for nx=%d TM=%d\n", s->nx, s->tx.lines);
+    else {
+      fprintf(stderr, "liblippmannschwinger: no three-line x kernel for nx=%d TM=%d\n", s->nx, s->tx.lines);
+      s->launch_failed = true;
+    }
 #undef LS_XH
     mark(s, 2);
     return;
@@ -767,6 +780,7 @@ int solve_impl(ls_solver* s, const float* d_lambda, const float* d_mu, const int
   s->tev.clear();
   s->tkind.clear();
   s->n_rebase = 0;
+  s->launch_failed = false;
   // reference medium per material: mid-range of the fields (lippmann_schwinger.cu:172-173)
   {
     dim3 grid(s->nblk_mm, s->nmat);
@@ -945,6 +959,7 @@ int solve_impl(ls_solver* s, const float* d_lambda, const float* d_mu, const int
   }
   LS_HIP(hipGetLastError());
   if (sync_state(s)) return -1;
+  if (s->launch_failed) return -1;
 
   // ---- outputs: eps = base + delta, sigma = C eps + tau, FP64 means ----
   {
@@ -1021,21 +1036,30 @@ std::vector<Tile> tile_candidates(const HostAxis& a, const std::vector<int>& wan
 
 template <typename F>
 float time_launches(ls_solver* s, int reps, F&& launch) {
-  hipEvent_t e0, e1;
-  if (hipEventCreate(&e0) != hipSuccess) return -1.f;
-  if (hipEventCreate(&e1) != hipSuccess) {
-    (void)hipEventDestroy(e0);
+  // a candidate whose launch is rejected (no kernel for the tile, too much LDS)
+  // is reported as -1 and skipped by the sweep
+  s->launch_failed = false;
+  (void)hipGetLastError();
+  launch();  // warm-up
+  if (s->launch_failed || hipGetLastError() != hipSuccess || hipStreamSynchronize(s->stream) != hipSuccess) {
+    (void)hipGetLastError();
+    s->launch_failed = false;
     return -1.f;
   }
-  launch();  // warm-up
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess) return -2.f;
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return -2.f;
+  }
   (void)hipEventRecord(e0, s->stream);
   for (int i = 0; i < reps; ++i) launch();
   (void)hipEventRecord(e1, s->stream);
-  float ms = -1.f;
+  float ms = -2.f;
   if (hipEventSynchronize(e1) == hipSuccess) (void)hipEventElapsedTime(&ms, e0, e1);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  return hipGetLastError() == hipSuccess ? ms / reps : -1.f;
+  return hipGetLastError() == hipSuccess ? ms / reps : -2.f;
 }
 
 // Time each candidate tile of the three FP32 passes on the solver's own
@@ -1074,10 +1098,15 @@ int autotune_impl(ls_solver* s, const float* lam, const float* mu, float* eps, i
       cur = t;
       apply_tiles(s);
       const float ms = time_launches(s, reps, launch);
+      if (ms == -1.f) {
+        if (verbose_tune) fprintf(stderr, "  tune %s: candidate rejected at launch\n", name);
+        continue;
+      }
       if (ms < 0.f) return -1;
       if (verbose_tune)
-        fprintf(stderr, "  tune %s: %s lines %d E %d NT %d lds %zu: %.4f ms\n", name,
-                t.reg == 2 ? "three-line" : (t.reg ? "register" : "lds"), t.lines, t.E, t.NT, t.lds, ms);
+        fprintf(stderr, "  tune %s: %s%s lines %d E %d NT %d lds %zu: %.4f ms\n", name,
+                t.reg == 2 ? "three-line" : (t.reg ? "register" : "lds"), t.inpl ? " in place" : "", t.lines, t.E,
+                t.NT, t.lds, ms);
       if (best_ms < 0.f || ms < best_ms) {
         best_ms = ms;
         best = t;
