@@ -134,6 +134,29 @@ struct AAParams {
   int record;           // x pass: store y of the newest residual
 };
 
+// The self-conjugate planes k_z = 0 and (nz even) k_z = nz/2.  There the
+// reference's xi~ (k in [0, n)) is odd under the in-plane flip (k_x, k_y) ->
+// (-k_x, -k_y), so r_hat is not Hermitian on those planes and the C2R
+// transform keeps its Hermitian part only: after the inverse y pass the
+// plane holds R(x, y) whose real part is the field and whose imaginary part
+// the C2R drops.  The Gram row from y counts |R|^2; the inverse y pass
+// records Im R of the two planes in a small ring, and k_aa_gram_planes
+// subtracts their contribution, so the Gram row is the real-space one.
+struct AAPlanes {
+  float* im;            // [R][m1][npl][nx][ny][6] (nullptr: not recording)
+  const int* head;      // [R] newest slot
+  long long per_slot;   // floats per slot = npl * nx * ny * 6
+  int m1;
+};
+
+__device__ __forceinline__ void aa_plane_record(const AAPlanes& pl, const GridDev& g, int r, int x, int y, int q,
+                                                float im) {
+  const int kz = q / 6, c = q - 6 * kz;
+  if (kz != 0 && 2 * kz != g.nz) return;
+  const int plane = kz == 0 ? 0 : 1;
+  pl.im[((long long)r * pl.m1 + pl.head[r]) * pl.per_slot + (((long long)plane * g.nx + x) * g.ny + y) * 6 + c] = im;
+}
+
 // y of one mode (see AAParams); q2 = |xi~|^2 > 0, inv = 1 / |xi~|
 __device__ __forceinline__ void aa_y(float wk, float q2, float inv, float n0, float n1, float n2, float2 u0,
                                      float2 u1, float2 u2, float2& y0, float2& y1, float2& y2) {
@@ -352,7 +375,7 @@ __global__ void __launch_bounds__(NT) k_zline(GridDev g, const RhsDesc* __restri
 // rsel >= 0: the FP64 base evaluation (spec holds that one load case only).
 template <bool INV, int E, int NT, typename V>
 __global__ void __launch_bounds__(NT) k_ypass(GridDev g, const RhsState* __restrict__ st, V* __restrict__ spec,
-                                              FastDiv f_T, int ntiles, int rsel) {
+                                              FastDiv f_T, int ntiles, int rsel, AAPlanes pl) {
   const int T = (int)f_T.d;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   V* lds = reinterpret_cast<V*>(lds_raw);
@@ -378,7 +401,12 @@ __global__ void __launch_bounds__(NT) k_ypass(GridDev g, const RhsState* __restr
   for (int e = tid; e < items; e += NT) {
     const int y = fdiv(e, f_T), t = e - y * T;
     const int q = q0 + t;
-    if (q < Q) S[(long long)y * Q + q] = lds[e];
+    if (q < Q) {
+      S[(long long)y * Q + q] = lds[e];
+      if constexpr (INV) {
+        if (pl.im) aa_plane_record(pl, g, r, x, y, q, (float)lds[e].y);
+      }
+    }
   }
 }
 
@@ -810,7 +838,7 @@ __global__ void __launch_bounds__(kThreads) k_minmax(const float* __restrict__ l
 // s = age (0 = newest), FP64 products and sums in a fixed order.  Grid
 // (nblk, R), `chunk` complex entries per block.
 __global__ void __launch_bounds__(kThreads) k_aa_gram(const RhsState* __restrict__ st, AAParams aa,
-                                                      double* __restrict__ dots, long long chunk) {
+                                                      double* __restrict__ dots, long long chunk, int stride) {
   __shared__ double red[kThreads / 64];
   const int r = blockIdx.y;
   if (!st[r].active) return;
@@ -836,7 +864,41 @@ __global__ void __launch_bounds__(kThreads) k_aa_gram(const RhsState* __restrict
   }
   for (int q = 0; q < aa.m1; ++q) {
     const double b = block_sum(acc[q], red);
-    if (threadIdx.x == 0) dots[((long long)r * gridDim.x + blockIdx.x) * aa.m1 + q] = (q < f) ? b : 0.0;
+    if (threadIdx.x == 0) dots[((long long)r * stride + blockIdx.x) * aa.m1 + q] = (q < f) ? b : 0.0;
+  }
+}
+
+// Correction of the Gram row for the self-conjugate planes (see AAPlanes):
+// dots[r][boff + blk][s] = -scale sum w_c Im R_new Im R^(s), scale = 1/(nx ny)
+// (the y ring counts (1/(nx ny)) sum_{x,y} |R|^2 per plane by Parseval).
+__global__ void __launch_bounds__(kThreads) k_aa_gram_planes(const RhsState* __restrict__ st, AAPlanes pl,
+                                                             const int* __restrict__ filled, double* __restrict__ dots,
+                                                             long long chunk, int stride, int boff, double scale) {
+  __shared__ double red[kThreads / 64];
+  const int r = blockIdx.y;
+  if (!st[r].active) return;
+  const int hd = pl.head[r], f = filled[r];
+  const float* base = pl.im + (long long)r * pl.m1 * pl.per_slot;
+  const long long i0 = (long long)blockIdx.x * chunk, i1 = min(pl.per_slot, i0 + chunk);
+  const float* yn = base + hd * pl.per_slot;
+  double acc[kMaxDepth1];
+#pragma unroll
+  for (int q = 0; q < kMaxDepth1; ++q) acc[q] = 0.0;
+  for (long long i = i0 + threadIdx.x; i < i1; i += kThreads) {
+    const int c = (int)(i % 6);
+    const double a = (c < 3 ? 1.0 : 2.0) * yn[i];
+#pragma unroll
+    for (int q = 0; q < kMaxDepth1; ++q) {
+      if (q < f) {
+        int sl = hd - q;
+        if (sl < 0) sl += pl.m1;
+        acc[q] += a * base[sl * pl.per_slot + i];
+      }
+    }
+  }
+  for (int q = 0; q < pl.m1; ++q) {
+    const double b = block_sum(acc[q], red);
+    if (threadIdx.x == 0) dots[((long long)r * stride + boff + blockIdx.x) * pl.m1 + q] = (q < f) ? -scale * b : 0.0;
   }
 }
 

@@ -32,7 +32,7 @@ struct RegLayout {
 // fixed x of one load case (same tiling as k_ypass).
 template <int A, int B, int T, int NT, bool INV>
 __global__ void __launch_bounds__(NT) k_ypass_r(GridDev g, const RhsState* __restrict__ st,
-                                                float2* __restrict__ spec, int ntiles) {
+                                                float2* __restrict__ spec, int ntiles, AAPlanes pl) {
   constexpr int N = A * B;
   using L = RegLayout<N, B>;
   extern __shared__ float2 lds[];
@@ -69,6 +69,12 @@ __global__ void __launch_bounds__(NT) k_ypass_r(GridDev g, const RhsState* __res
     if (l < nq) {
 #pragma unroll
       for (int k2 = 0; k2 < B; ++k2) S[(long long)(k1 + A * k2) * Q + l] = u[k2];
+      if constexpr (INV) {
+        if (pl.im) {
+#pragma unroll
+          for (int k2 = 0; k2 < B; ++k2) aa_plane_record(pl, g, r, x, k1 + A * k2, q0 + l, u[k2].y);
+        }
+      }
     }
   }
 }

@@ -218,6 +218,10 @@ struct ls_solver {
   // Anderson
   float* aaG = nullptr;
   float2* aaU = nullptr;
+  float* aaPl = nullptr;    // Im R of the self-conjugate planes after the inverse y pass
+  long long pl_slot = 0;    // floats per slot
+  int nblk_p = 0;           // blocks per load case of the plane correction
+  long long chunk_p = 0;
   double* aa_alpha = nullptr;
   double* aa_dots = nullptr;
   double* aa_gram = nullptr;
@@ -358,16 +362,23 @@ void zline(ls_solver* s, const float* lam, const float* mu, float* eps) {
   mark(s, MODE == Z_AA ? 4 : 0);
 }
 
-void ypass(ls_solver* s, bool inv) {
+// y pass over all active load cases; record (inverse only, Anderson): store
+// Im R of the self-conjugate planes for the Gram correction.
+void ypass(ls_solver* s, bool inv, bool record = false) {
+  AAPlanes pl;
+  pl.im = (record && inv && s->depth > 0) ? s->aaPl : nullptr;
+  pl.head = s->aa_head;
+  pl.per_slot = s->pl_slot;
+  pl.m1 = s->m1;
   const unsigned grid = (unsigned)s->nblk_y * s->nrhs;
   const size_t lds = s->ty.lds;
   if (s->ty.reg) {
 #define LS_YR(NV, AV, BV, LV, NTV)                                                                      \
   else if (s->ny == NV && s->ty.lines == LV && s->ty.NT == NTV && s->ty.E == AV) {                     \
     if (inv)                                                                                            \
-      k_ypass_r<AV, BV, LV, NTV, true><<<grid, NTV, lds, s->stream>>>(s->gd, s->d_st, s->spec, s->ntiles_y);  \
+      k_ypass_r<AV, BV, LV, NTV, true><<<grid, NTV, lds, s->stream>>>(s->gd, s->d_st, s->spec, s->ntiles_y, pl); \
     else                                                                                                \
-      k_ypass_r<AV, BV, LV, NTV, false><<<grid, NTV, lds, s->stream>>>(s->gd, s->d_st, s->spec, s->ntiles_y); \
+      k_ypass_r<AV, BV, LV, NTV, false><<<grid, NTV, lds, s->stream>>>(s->gd, s->d_st, s->spec, s->ntiles_y, pl); \
   }
     if (false) {
     }
@@ -383,9 +394,9 @@ void ypass(ls_solver* s, bool inv) {
   const FastDiv fT = make_fastdiv(s->ty.lines);
 #define LS_Y(EV, NTV)                                                                                               \
   if (inv)                                                                                                          \
-    k_ypass<true, EV, NTV, float2><<<grid, NTV, lds, s->stream>>>(s->gd, s->d_st, s->spec, fT, s->ntiles_y, -1);   \
+    k_ypass<true, EV, NTV, float2><<<grid, NTV, lds, s->stream>>>(s->gd, s->d_st, s->spec, fT, s->ntiles_y, -1, pl); \
   else                                                                                                              \
-    k_ypass<false, EV, NTV, float2><<<grid, NTV, lds, s->stream>>>(s->gd, s->d_st, s->spec, fT, s->ntiles_y, -1)
+    k_ypass<false, EV, NTV, float2><<<grid, NTV, lds, s->stream>>>(s->gd, s->d_st, s->spec, fT, s->ntiles_y, -1, pl)
   LS_DISPATCH32(s->ty, LS_Y)
 #undef LS_Y
   mark(s, 1);
@@ -446,9 +457,10 @@ void base_eval_one(ls_solver* s, int r, const float* lam, const float* mu, const
   LS_DISPATCH64(s->tz64, LS_BZ)
 #undef LS_BZ
   const FastDiv fT = make_fastdiv(s->ty64.lines);
+  const AAPlanes nopl = {nullptr, nullptr, 0, 1};
 #define LS_BY(EV, NTV)                                                                                        \
   k_ypass<false, EV, NTV, double2><<<s->nblk_y64, NTV, s->ty64.lds, s->stream>>>(s->gd, s->d_st, s->spec64,   \
-                                                                                fT, s->ntiles_y64, r)
+                                                                                fT, s->ntiles_y64, r, nopl)
   LS_DISPATCH64(s->ty64, LS_BY)
 #undef LS_BY
   const FastDiv fTm = make_fastdiv(s->tx64.lines), fcols = make_fastdiv(6 * s->tx64.lines);
@@ -514,7 +526,7 @@ void free_solver(ls_solver* s) {
     (void)hipFree(a.d_ph);
   }
   void* dev[] = {s->spec,    s->spec64,  s->base,    s->dbase,    s->partials, s->dc,       s->d_rhs,
-                 s->d_st,    s->d_sel,   s->part6,   s->d_mean6,  s->d_mm,     s->aaG,      s->aaU,
+                 s->d_st,    s->d_sel,   s->part6,   s->d_mean6,  s->d_mm,     s->aaG,      s->aaU,      s->aaPl,
                  s->aa_alpha, s->aa_dots, s->aa_gram, s->aa_head, s->aa_filled};
   for (void* p : dev) (void)hipFree(p);
   (void)hipHostFree(s->h_st);
@@ -654,8 +666,13 @@ int create_impl(ls_solver* s, const int* voxels, const float* extents, int nrhs,
     // y ring in the d_base layout; Gram-row partials per k_aa_gram block
     s->nblk_g = (int)std::min<long long>(512, (s->dl.per_rhs + 16383) / 16384);
     s->chunk_g = ((s->dl.per_rhs + s->nblk_g - 1) / s->nblk_g + 1) / 2 * 2;
+    // self-conjugate planes: k_z = 0, and k_z = nz/2 for even nz
+    s->pl_slot = (long long)(s->nz % 2 == 0 ? 2 : 1) * s->nx * s->ny * 6;
+    s->nblk_p = (int)std::min<long long>(64, (s->pl_slot + 16383) / 16384);
+    s->chunk_p = (s->pl_slot + s->nblk_p - 1) / s->nblk_p;
     if (dalloc(&s->aaG, hist) || dalloc(&s->aaU, (size_t)nrhs * s->m1 * s->dl.per_rhs) ||
-        dalloc(&s->aa_alpha, (size_t)nrhs * s->m1) || dalloc(&s->aa_dots, (size_t)nrhs * s->nblk_g * s->m1) ||
+        dalloc(&s->aaPl, (size_t)nrhs * s->m1 * s->pl_slot) || dalloc(&s->aa_alpha, (size_t)nrhs * s->m1) ||
+        dalloc(&s->aa_dots, (size_t)nrhs * (s->nblk_g + s->nblk_p) * s->m1) ||
         dalloc(&s->aa_gram, (size_t)nrhs * s->m1 * s->m1) ||
         dalloc(&s->aa_head, nrhs) || dalloc(&s->aa_filled, nrhs))
       return -1;
@@ -674,6 +691,7 @@ int reset_aa(ls_solver* s) {
   LS_HIP(hipMemcpyAsync(s->aa_filled, filled.data(), filled.size() * sizeof(int), hipMemcpyHostToDevice, s->stream));
   // entries of the y ring outside the grid's modes are never written: keep them zero
   LS_HIP(hipMemsetAsync(s->aaU, 0, (size_t)s->nrhs * s->m1 * s->dl.per_rhs * sizeof(float2), s->stream));
+  LS_HIP(hipMemsetAsync(s->aaPl, 0, (size_t)s->nrhs * s->m1 * s->pl_slot * sizeof(float), s->stream));
   LS_HIP(hipStreamSynchronize(s->stream));
   return 0;
 }
@@ -685,23 +703,30 @@ void init_fields(ls_solver* s, const int* d_sel, float* delta, const float* src)
 
 // Residual evaluation of the current iterate from the spectrum after the
 // forward z and y passes: x pass (residual, Gamma0) and stop test.  With
-// Anderson the x pass records the residual and its Gram row (in a new ring
-// slot if `advance`, else replacing the newest entry, e.g. after a rebase) and
-// the weights of the next z pass are solved.
+// Anderson the x pass records the residual's y form (in a new ring slot if
+// `advance`, else replacing the newest entry, e.g. after a rebase); the
+// weights follow the inverse y pass of the next iteration (aa_weights).
 void eval_x(ls_solver* s, double rtol, double atol, int mode, bool advance) {
   if (s->depth > 0 && advance)
     k_aa_advance<<<(s->nrhs + 63) / 64, 64, 0, s->stream>>>(s->d_st, s->aa_head, s->aa_filled, s->aa_gram, s->nrhs,
                                                            s->m1);
   xpass(s, 1, true, s->depth > 0);
   check(s, rtol, atol, mode);
-  if (s->depth > 0) {
-    k_aa_gram<<<dim3((unsigned)s->nblk_g, (unsigned)s->nrhs), kThreads, 0, s->stream>>>(s->d_st, aa_params(s, 1),
-                                                                                      s->aa_dots, s->chunk_g);
-    mark(s, 5);
-    k_aa_solve<<<s->nrhs, kThreads, 0, s->stream>>>(s->d_st, s->aa_dots, s->nblk_g, s->m1, s->aa_gram,
-                                                    s->aa_alpha, s->aa_head, s->aa_filled);
-    mark(s, 3);
-  }
+}
+
+// Anderson weights of the next z pass from the newest record (x pass y ring +
+// plane correction from the inverse y pass): Gram row, then the solve.
+void aa_weights(ls_solver* s) {
+  const int stride = s->nblk_g + s->nblk_p;
+  k_aa_gram<<<dim3((unsigned)s->nblk_g, (unsigned)s->nrhs), kThreads, 0, s->stream>>>(s->d_st, aa_params(s, 1),
+                                                                                    s->aa_dots, s->chunk_g, stride);
+  AAPlanes pl = {s->aaPl, s->aa_head, s->pl_slot, s->m1};
+  k_aa_gram_planes<<<dim3((unsigned)s->nblk_p, (unsigned)s->nrhs), kThreads, 0, s->stream>>>(
+      s->d_st, pl, s->aa_filled, s->aa_dots, s->chunk_p, stride, s->nblk_g, 1.0 / ((double)s->nx * s->ny));
+  mark(s, 5);
+  k_aa_solve<<<s->nrhs, kThreads, 0, s->stream>>>(s->d_st, s->aa_dots, stride, s->m1, s->aa_gram, s->aa_alpha,
+                                                  s->aa_head, s->aa_filled);
+  mark(s, 3);
 }
 
 // One residual evaluation after a z-line forward pass: y fwd, then eval_x.
@@ -714,11 +739,13 @@ void evaluate(ls_solver* s, double rtol, double atol, int mode) {
 // the x pass of the previous check).
 void iterate(ls_solver* s, const float* lam, const float* mu, float* eps, double rtol, double atol,
              bool evaluate_after) {
-  ypass(s, true);
-  if (s->depth == 0)
+  ypass(s, true, s->depth > 0);
+  if (s->depth == 0) {
     zline<Z_UPD>(s, lam, mu, eps);
-  else
+  } else {
+    aa_weights(s);
     zline<Z_AA>(s, lam, mu, eps);
+  }
   if (evaluate_after) evaluate(s, rtol, atol, 0);
 }
 
