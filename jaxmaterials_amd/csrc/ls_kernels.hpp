@@ -885,7 +885,7 @@ __global__ void __launch_bounds__(kThreads) k_aa_gram_planes(const RhsState* __r
 #pragma unroll
   for (int q = 0; q < kMaxDepth1; ++q) acc[q] = 0.0;
   for (long long i = i0 + threadIdx.x; i < i1; i += kThreads) {
-    const int c = (int)(i % 6);
+    const int c = (int)i % 6;  // per_slot < 2^31
     const double a = (c < 3 ? 1.0 : 2.0) * yn[i];
 #pragma unroll
     for (int q = 0; q < kMaxDepth1; ++q) {
