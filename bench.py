@@ -119,6 +119,137 @@ def cpu_baseline_sample(lam, mu, gs, gpu_iters, timed=2):
     }
 
 
+DATASET_METRIC = ("dataset samples/sec: reference layered-fibre generator at 160x160x64 "
+                  "(placement + rasterisation + C_eff at rtol = atol = 1e-4)")
+
+
+def _dataset_generator(gs, n_samples, seed):
+    """The reference's dataset script (src/generate_data.py:14-55): 0.5 x 0.5 x 0.2
+    domain, 3 fibre layers between 0.01-thick void slabs, volume fraction 0.3,
+    radii from the clipped normal law (7.5e-3 +- 0.5e-3 in [5e-3, 1e-2])."""
+    import numpy as np
+
+    from jaxmaterials_amd.data.fibres import FibreRadiusDistribution, LayeredFibresDatasetGenerator
+
+    rd = FibreRadiusDistribution(r_avg=7.5e-3, r_min=5.0e-3, r_max=10.0e-3, sigma=0.5e-3, gaussian=True)
+    return LayeredFibresDatasetGenerator(gs, 3, 0.01, n_samples, rd, volume_fraction=0.3, mu_fibre=0.5,
+                                         mu_material=1.0, mu_void=0.1, lambda_fibre=0.1, lambda_material=0.5,
+                                         lambda_void=1.0, dtype=np.float32, rng=np.random.default_rng(seed))
+
+
+def dataset_main(args):
+    """One step = generate() of --samples samples of the reference's dataset
+    workload: placement on the host, GPU rasterisation, all load cases of up
+    to --batch samples per plain-LS solver call (the reference's iteration),
+    fields and tensors copied to host arrays as the reference's generate()
+    leaves them."""
+    os.environ["LS_TIMING"] = "1"
+    import numpy as np
+    import torch
+
+    from jaxmaterials_amd.common import GridSpec
+
+    gs = GridSpec(160, 160, 64, 0.5, 0.5, 0.2)
+    N = gs.nx * gs.ny * gs.nz
+    M = gs.nx * gs.ny * (gs.nz // 2 + 1)
+    for w in range(args.warmup):
+        _dataset_generator(gs, min(args.samples, args.batch), 1000 + w).generate(max_batch=args.batch)
+    times, gens = [], []
+    for step in range(args.steps):
+        gen = _dataset_generator(gs, args.samples, 5713853 + step)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        gen.generate(max_batch=args.batch)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        gens.append(gen)
+    sec = float(np.mean(times))
+    total = {k: float(np.mean([g.timing[k] for g in gens])) for k in ("placement_s", "rasterise_s", "solve_s",
+                                                                       "copy_s")}
+    # algorithmic bytes of the solver passes, batch by batch (padding slots repeat the last sample)
+    kinds = {}
+    k = gens[0].timing["per_call"]
+    for name in ("z", "y", "x"):
+        b = 0
+        for g in gens:
+            its = g.iterations
+            for c0 in range(0, len(its), k):
+                chunk = [its[min(c0 + q, len(its) - 1)] for q in range(k)]
+                flat = [int(i) for row in chunk for i in row]
+                b += model_bytes(name, N, M, flat, 0, 0) + (
+                    sum(max(int(i) for i in row) for row in chunk) - max(flat)) * 8 * N * (name == "z")
+        ms = sum(g.timing["solver_ms"].get(name, 0.0) for g in gens)
+        launches = sum(g.timing["solver_launches"].get(name, 0) for g in gens)
+        kinds[name] = {"ms_total": ms / len(gens), "launches": launches // len(gens),
+                       "algorithmic_bytes": b / len(gens), "achieved_GBps": b / (ms / 1e3) / 1e9 if ms else None}
+    dom = max(kinds, key=lambda q: kinds[q]["ms_total"])
+    dk = kinds[dom]
+    iters = np.concatenate([g.iterations.reshape(-1) for g in gens])
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = dataset_cpu_baseline(gs, gens[0], float(np.mean(iters)))
+    line = {
+        "metric": DATASET_METRIC,
+        "value": args.samples / sec,
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sec * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": f"reference dataset script: 160x160x64 grid of 0.5 x 0.5 x 0.2, 3 layers, "
+                               f"void 0.01, vf 0.3, radii 7.5e-3 +- 0.5e-3; {args.samples} samples per step, "
+                               f"six load cases each, plain LS to rtol = atol = 1e-4, maxiter 100, "
+                               f"{k} samples per solver call",
+                   "grid": [160, 160, 64], "samples_per_step": args.samples},
+        "phases_s_per_step": total,
+        "iterations_mean": float(np.mean(iters)),
+        "roofline": {"bound": "hbm", "kernel": {"z": "k_zline<Z_UPD>", "y": "k_ypass", "x": "k_xpass"}[dom],
+                     "achieved": dk["achieved_GBps"], "peak": 8000.0, "unit": "GB/s",
+                     "frac": dk["achieved_GBps"] / 8000.0 if dk["achieved_GBps"] else None, "traffic": None,
+                     "passes": kinds},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+def dataset_cpu_baseline(gs, gen, mean_iters):
+    """The reference's host cost of one sample: its NumPy rasteriser (the
+    oracle's literal restatement of data.py:267-314) timed on one sample, plus
+    the FP64 oracle plain LS timed for two iterations of one load case and
+    extrapolated to six load cases at the GPU's mean iteration count.  The
+    placement (about 10 ms of compiled code per sample) is left out."""
+    import numpy as np
+
+    from oracle import fibres_oracle as fo
+    from oracle import ls_oracle as O
+
+    workers = len(os.sched_getaffinity(0))
+    ref = _dataset_generator(gs, 1, 5713853)
+    _, pos, rad, ori = ref.generate_fibre_positions()
+    t0 = time.perf_counter()
+    f = fo.material_properties(gs, ref.d_void, pos, rad, ori, (ref.mu_void, ref.mu_fibre, ref.mu_material),
+                               (ref.lambda_void, ref.lambda_fibre, ref.lambda_material), dtype=np.float32)
+    t_raster = time.perf_counter() - t0
+    lam64, mu64 = f[0].astype(np.float64), f[1].astype(np.float64)
+    e = np.zeros(6)
+    e[0] = 1.0
+    O.lippmann_schwinger_rfft(lam64, mu64, e, gs, rtol=0.0, depth=0, maxiter=1, workers=workers)
+    t0 = time.perf_counter()
+    O.lippmann_schwinger_rfft(lam64, mu64, e, gs, rtol=0.0, depth=0, maxiter=2, workers=workers)
+    s_it = (time.perf_counter() - t0) / 2.5
+    s_sample = t_raster + 6 * mean_iters * s_it
+    return {"value": 1.0 / s_sample, "unit": "samples/s", "cores": workers, "kind": "port",
+            "sample": (f"one sample: NumPy rasteriser {t_raster:.2f} s (1 thread) + FP64 NumPy/SciPy plain LS "
+                       f"{s_it:.3f} s/iteration ({workers} threads) x 6 load cases x {mean_iters:.1f} iterations "
+                       f"(extrapolated from 2 timed iterations)"),
+            "rasterise_s": t_raster, "s_per_iteration": s_it, "extrapolated": True}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,11 +260,19 @@ def main():
     ap.add_argument("--methods", default="plain,aa,aa_ws")
     ap.add_argument("--rtol", type=float, default=1e-6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=("crossply256", "dataset"), default="crossply256",
+                    help="crossply256: the C_eff solve benchmark (default); dataset: the reference's "
+                         "layered-fibre dataset generation")
+    ap.add_argument("--samples", type=int, default=128, help="dataset workload: samples per step")
+    ap.add_argument("--batch", type=int, default=32, help="dataset workload: samples per solver call")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the solver runs each solve on a single MI355X and this build has no "
               "multi-GPU path (--gpus must be 1).", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "dataset":
+        dataset_main(args)
+        return
 
     os.environ["LS_TIMING"] = "1"
     import numpy as np

@@ -135,6 +135,25 @@ int ls_crossply_place(const int *voxels, const float *extents, int nlayers,
                       double radius, double vf, unsigned long long seed,
                       double *out, int cap);
 
+/* Lame fields of the reference's layered-fibre generator, rasterised on the
+ * GPU (LayeredFibresDatasetGenerator.material_properties,
+ * src/jaxmaterials/data/data.py:267-314; the reference evaluates it with
+ * NumPy on the host).  voxels = {nx, ny, nz}; extents = {Lx, Ly, Lz} in
+ * float64 (the reference's grid spec holds Python floats).  fibres[n][4] =
+ * (in-plane coordinate p0, z coordinate p1, radius r, orientation): every
+ * fibre of every layer, boundary copies included, as generate_fibre_positions
+ * returns them (data.py:188-265); orientation 0 tests p0 against x, 1
+ * against y.  mu3 / lambda3 = (void, fibre, matrix) values.  d_out = device
+ * [2][nx][ny][nz] of float32 (dtype_bytes 4) or float64 (8): channel 0 mu,
+ * channel 1 lambda; bit-identical to the reference in both dtypes.
+ * stream = hipStream_t (NULL: default stream); returns after the fields are
+ * written.  0 on success, -1 on error. */
+int ls_rasterise_layered_fibres(const int *voxels, const double *extents,
+                                double d_void, int nfibres,
+                                const double *fibres, const double *mu3,
+                                const double *lambda3, int dtype_bytes,
+                                void *d_out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -52,6 +52,22 @@ SIGNATURES = {
         ctypes.c_int,
         [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     ),
+    "ls_rasterise_layered_fibres": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+         ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p],
+    ),
+    # reference fibre movement (include/fibres.h)
+    "dist_periodic": (
+        None,
+        [ctypes.c_uint, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+    ),
+    "initialise_rng": (None, [ctypes.c_uint, ctypes.POINTER(ctypes.c_uint), ctypes.c_char_p]),
+    "move_fibres": (
+        None,
+        [ctypes.c_uint, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint, ctypes.c_double,
+         ctypes.c_uint, ctypes.POINTER(ctypes.c_uint), ctypes.c_char_p],
+    ),
     # learned warm start (include/ls_warmstart.h)
     "ls_ws_conv3d": (
         ctypes.c_int,

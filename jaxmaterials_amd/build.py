@@ -1,8 +1,9 @@
 """Build recipe of liblippmannschwinger.so (gfx950, in-tree).
 
 ``python -m jaxmaterials_amd.build`` compiles every HIP source of the library
-into ``jaxmaterials_amd/liblippmannschwinger.so``.  hipcc cross-compiles for
-gfx950 without a GPU.
+(hipcc) and its host-only C++ sources (g++) into
+``jaxmaterials_amd/liblippmannschwinger.so``.  hipcc cross-compiles for gfx950
+without a GPU.
 """
 
 import os
@@ -17,6 +18,12 @@ LIB_PATH = os.path.join(HERE, LIB_NAME)
 
 # every HIP source of the library; each compiles to its own object in parallel
 SOURCES = ["ls_solver.hip", "microstructure.hip", "conv3d.hip", "derivatives.hip"]
+# host-only C++ sources, compiled by g++ (the reference fibre movement is
+# compiled by g++ in the reference too, and its results depend on the
+# standard library's random engines and distributions)
+HOST_SOURCES = ["fibres.cpp"]
+# (-fno-math-errno lets sqrt vectorise; it changes no result)
+GXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-math-errno"]
 
 HIPCC_FLAGS = [
     "-O3",
@@ -37,6 +44,13 @@ def hipcc():
     exe = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
     if not os.path.exists(exe):
         raise RuntimeError("hipcc not found; ROCm is required to build liblippmannschwinger.so")
+    return exe
+
+
+def gxx():
+    exe = shutil.which("g++")
+    if not exe:
+        raise RuntimeError("g++ not found; it builds the host sources of liblippmannschwinger.so")
     return exe
 
 
@@ -62,6 +76,13 @@ def build(verbose=False):
         objs.append(obj)
         flags = [f for f in HIPCC_FLAGS if f != "-shared"]
         procs.append((s, _run([hipcc()] + flags + ["-c", "-o", obj, os.path.join(CSRC, s)], verbose)))
+    for s in HOST_SOURCES:
+        src = os.path.join(CSRC, s)
+        if not os.path.exists(src):
+            continue
+        obj = os.path.join(objdir, os.path.splitext(s)[0] + ".o")
+        objs.append(obj)
+        procs.append((s, _run([gxx()] + GXX_FLAGS + ["-c", "-o", obj, src], verbose)))
     failed = []
     for s, p in procs:
         out, _ = p.communicate()

@@ -127,14 +127,16 @@ struct Tile {
 #define LS_REG_Y_LIST(F)                                                                         \
   F(256, 16, 16, 16, 256) F(256, 16, 16, 8, 128) F(128, 16, 8, 32, 256) F(128, 16, 8, 48, 384)  \
   F(96, 12, 8, 32, 256) F(96, 12, 8, 64, 512) F(64, 8, 8, 32, 256) F(64, 8, 8, 64, 512)         \
-  F(48, 6, 8, 32, 256) F(48, 6, 8, 64, 512) F(32, 8, 4, 64, 256) F(32, 8, 4, 128, 512)
+  F(48, 6, 8, 32, 256) F(48, 6, 8, 64, 512) F(32, 8, 4, 64, 256) F(32, 8, 4, 128, 512)          \
+  F(160, 5, 32, 32, 256) F(160, 32, 5, 32, 160) F(160, 32, 5, 64, 320) F(160, 5, 32, 64, 320)
 // x pass with three-line transforms (k_xpass_h): (n, A, B, TM, NT, in place)
 #define LS_REG_H_LIST(F)                                                                                     \
   F(256, 32, 8, 8, 256, 1) F(256, 32, 8, 4, 256, 0) F(256, 16, 16, 4, 256, 0) F(256, 32, 8, 4, 128, 0)      \
   F(256, 32, 8, 4, 128, 1) F(128, 16, 8, 8, 256, 0) F(128, 16, 8, 4, 256, 0) F(128, 16, 8, 8, 256, 1)        \
   F(96, 12, 8, 8, 256, 0) F(96, 12, 8, 4, 128, 0) F(96, 12, 8, 8, 256, 1) F(64, 8, 8, 8, 256, 0)             \
   F(64, 8, 8, 16, 512, 0) F(64, 8, 8, 16, 512, 1) F(48, 6, 8, 16, 256, 0) F(48, 6, 8, 8, 256, 0)             \
-  F(32, 8, 4, 16, 256, 0) F(32, 8, 4, 32, 512, 0)
+  F(32, 8, 4, 16, 256, 0) F(32, 8, 4, 32, 512, 0) F(160, 32, 5, 8, 256, 1) F(160, 32, 5, 8, 256, 0)          \
+  F(160, 32, 5, 16, 512, 1) F(160, 32, 5, 4, 128, 0)
 
 struct RegCfg {
   int n, A, B, lines, NT, inpl;
@@ -610,8 +612,15 @@ int create_impl(ls_solver* s, const int* voxels, const float* extents, int nrhs,
   if (env_int("LS_REG_FFT", 1)) {
     const std::vector<Tile> ry = reg_tiles(kRegY, sizeof(kRegY) / sizeof(kRegY[0]), s->ny, 1);
     const std::vector<Tile> rh = reg_tiles(kRegH, sizeof(kRegH) / sizeof(kRegH[0]), s->nx, 2);
-    if (!ry.empty()) s->ty = ry[0];
-    if (!rh.empty()) s->tx = rh[0];
+    // LS_REG_Y_INDEX / LS_REG_X_INDEX select another compiled configuration
+    // of the length than its default (the first)
+    const int iy = env_int("LS_REG_Y_INDEX", 0), ix = env_int("LS_REG_X_INDEX", 0);
+    if ((!ry.empty() && (iy < 0 || iy >= (int)ry.size())) || (!rh.empty() && (ix < 0 || ix >= (int)rh.size()))) {
+      fprintf(stderr, "liblippmannschwinger: LS_REG_Y_INDEX / LS_REG_X_INDEX out of range\n");
+      return -1;
+    }
+    if (!ry.empty()) s->ty = ry[iy];
+    if (!rh.empty()) s->tx = rh[ix];
   }
   if (s->tx.lines > kMaxTm) {
     fprintf(stderr, "liblippmannschwinger: x tile of %d modes exceeds %d\n", s->tx.lines, kMaxTm);

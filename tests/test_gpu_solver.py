@@ -245,6 +245,56 @@ def test_register_fft_lengths_match_oracle(gpu, shape):
         s.close()
 
 
+def _register_configs():
+    """(kind, n, index) of every compiled register FFT configuration, read from
+    the LS_REG_Y_LIST / LS_REG_H_LIST tables of the solver source (the index
+    counts the configurations of one length in table order)."""
+    import os
+    import re
+
+    src = open(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "jaxmaterials_amd",
+                            "csrc", "ls_solver.hip")).read()
+    out = []
+    for kind, macro in (("y", "LS_REG_Y_LIST"), ("x", "LS_REG_H_LIST")):
+        body = re.search(r"#define " + macro + r"\(F\)((?:[^\n]*\\\n)*[^\n]*)", src).group(1)
+        seen = {}
+        for m in re.finditer(r"F\((\d+),", body):
+            n = int(m.group(1))
+            out.append((kind, n, seen.get(n, 0)))
+            seen[n] = seen.get(n, 0) + 1
+    return out
+
+
+REG_CONFIGS = _register_configs()
+
+
+def test_register_config_table_parsed():
+    assert len(REG_CONFIGS) >= 30
+    assert ("y", 160, 0) in REG_CONFIGS and ("x", 256, 0) in REG_CONFIGS
+
+
+@pytest.mark.parametrize("cfg", REG_CONFIGS, ids=lambda c: f"{c[0]}{c[1]}_{c[2]}")
+def test_every_register_config_matches_oracle(gpu, cfg, monkeypatch):
+    """Each compiled register configuration (not only a length's default),
+    selected by LS_REG_Y_INDEX / LS_REG_X_INDEX, against the FP64 oracle after
+    six plain and six Anderson(3) updates."""
+    kind, n, idx = cfg
+    shape = (10, n, 8) if kind == "y" else (n, 10, 8)
+    monkeypatch.setenv("LS_REG_Y_INDEX" if kind == "y" else "LS_REG_X_INDEX", str(idx))
+    gs = GridSpec(*shape, 1.0, 0.9, 0.3)
+    lmbda, mu = random_material(gs, seed=29, lo=0.5, hi=1.5)
+    lam_t = torch.from_numpy(lmbda).to(gpu)
+    mu_t = torch.from_numpy(mu).to(gpu)
+    for depth in (0, 3):
+        s = LSSolver(gs, nrhs=1, depth=depth)
+        out = s.solve(lam_t, mu_t, EPS_BAR[None].astype(np.float32), rtol=1e-20, atol=0.0, maxiter=6, check=False)
+        e_k, _, _ = oracle_fixed_updates(lmbda, mu, EPS_BAR, gs, 6, depth=depth)
+        err = rel_l2(out["epsilon"][0].cpu().numpy(), e_k)
+        print(f"register {kind} n={n} config {idx} depth {depth}: strain relL2 {err:.2e}", flush=True)
+        assert err <= 1e-5
+        s.close()
+
+
 def test_effective_tensor_256_against_fp64_oracle(gpu):
     """The benchmark workload: 256^3 cross-ply (16 layers of 16 voxels, fibre
     radius 6, RSA vf 0.5, seed 2), six load cases batched, Anderson depth 5 and
