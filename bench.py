@@ -390,6 +390,17 @@ def main():
                 "algorithmic_bytes_per_launch": dk["algorithmic_bytes"] / max(1, dk["launches"]),
                 "avg_launch_ms": dk["avg_launch_ms"],
                 "loop_achieved": best["loop_GBps"], "loop_frac": best["loop_GBps"] / peak}
+    # measured streaming ceilings of this GPU (grid-stride float4 kernels)
+    import ctypes
+
+    from jaxmaterials_amd import _lib
+
+    ceil = np.zeros(4)
+    torch.cuda.synchronize()
+    if _lib.load().ls_hbm_stream_probe(4 << 30, 5, ceil.ctypes.data_as(ctypes.c_void_p), None) == 0:
+        roofline["stream_ceiling_GBps"] = {"copy": ceil[0], "read": ceil[1], "write": ceil[2],
+                                           "read2_write1": ceil[3]}
+        roofline["frac_of_read2_write1_ceiling"] = dk["achieved_GBps"] / ceil[3]
     cpu = None
     if not args.no_cpu_baseline and "plain" in results:
         cpu = cpu_baseline_sample(lam.cpu().numpy(), mu.cpu().numpy(), gs, results["plain"]["iters"])

@@ -154,6 +154,13 @@ int ls_rasterise_layered_fibres(const int *voxels, const double *extents,
                                 const double *lambda3, int dtype_bytes,
                                 void *d_out, void *stream);
 
+/* Measured HBM streaming ceilings of the current device (a measurement aid,
+ * no counterpart in the reference): grid-stride float4 kernels over
+ * `bytes`-sized arrays (three are allocated), best of `reps` runs each;
+ * gbps[4] = GB/s of copy (read + write counted), read only, write only, and
+ * two reads + one write.  0 on success, -1 on error. */
+int ls_hbm_stream_probe(long long bytes, int reps, double *gbps, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

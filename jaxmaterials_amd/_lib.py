@@ -57,6 +57,7 @@ SIGNATURES = {
         [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
          ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p],
     ),
+    "ls_hbm_stream_probe": (ctypes.c_int, [ctypes.c_longlong, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
     # reference fibre movement (include/fibres.h)
     "dist_periodic": (
         None,

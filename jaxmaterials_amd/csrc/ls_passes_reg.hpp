@@ -16,6 +16,8 @@
 // pairing), stage 2 of the x pass walks consecutive k1 (stride B + 1, odd).
 //
 #pragma once
+#include <type_traits>
+
 #include "fft_reg.hpp"
 #include "ls_kernels.hpp"
 
@@ -26,6 +28,26 @@ struct RegLayout {
   static constexpr int Np = N + N / B + 1;
   static __device__ __forceinline__ int at(int l, int pos) { return l * Np + pos + pos / B; }
 };
+
+// The same accesses without padding: within each aligned group of 32
+// positions of line l, position pos is stored at pos ^ ((l ^ (pos / 32)) & 31)
+// restricted so that (pos / 32) only flips the low log2(B) bits.  With
+// pos = t + B k1: the 32 lanes of a stage-2 access (k1 varying) hit 32
+// different banks, and the lanes of a stage-1/3 access (lines l varying)
+// spread over the banks as with the padded layout, in N complex per line
+// instead of N + N / B + 1.  Needs N % 32 == 0 and B a power of two <= 32.
+template <int N, int B>
+struct SwzLayout {
+  static_assert(N % 32 == 0 && (B & (B - 1)) == 0 && B <= 32, "swizzled layout needs N % 32 == 0, B = 2^k <= 32");
+  static __device__ __forceinline__ int at(int l, int pos) {
+    return l * N + (pos ^ ((l ^ ((pos >> 5) & (B - 1))) & 31));
+  }
+};
+
+template <int N, int B>
+constexpr bool swz_ok() {
+  return N % 32 == 0 && (B & (B - 1)) == 0 && B <= 32;
+}
 
 // ---------------------------------------------------------------------------
 // y pass: FFT along y (ny = A B) of T consecutive (kz, component) columns at
@@ -117,7 +139,8 @@ __global__ void __launch_bounds__(NT) k_xpass_h(GridDev g, const RhsDesc* __rest
                                                 int do_gamma, AAParams aa) {
   constexpr int N = A * B;
   constexpr int LH = 3 * TM;  // h / g lines
-  using L = RegLayout<N, B>;
+  // in place: the transforms share the h / g lines' N LH complex, unpadded
+  using L = std::conditional_t<INPL && swz_ok<N, B>(), SwzLayout<N, B>, RegLayout<N, B>>;
   static_assert(!INPL || B * LH <= NT, "in-place stages 1 and 3 need one item per thread");
   extern __shared__ float2 lds[];
   float2* H = lds;                       // [x][LH]

@@ -162,7 +162,10 @@ std::vector<Tile> reg_tiles(const RegCfg* list, size_t count, int n, int kind) {
     t.E = c.A;  // distinguishes the (A, B) splits of one length
     t.inpl = c.inpl;
     const int np = c.n + c.n / c.B + 1;
-    t.lds = kind == 2 ? (size_t)3 * c.lines * (c.inpl ? np : c.n + np) * sizeof(float2)
+    // the in-place x kernel stores its transforms unpadded (swizzled) where the
+    // length allows it (SwzLayout)
+    const bool swz = c.n % 32 == 0 && (c.B & (c.B - 1)) == 0 && c.B <= 32;
+    t.lds = kind == 2 ? (size_t)3 * c.lines * (c.inpl ? (swz ? c.n : np) : c.n + np) * sizeof(float2)
                       : (size_t)c.lines * np * sizeof(float2);
     out.push_back(t);
   }
