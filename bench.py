@@ -35,7 +35,8 @@ def model_bytes(kind, N, M, iters, depth, nrebase=0):
       z Anderson : spectrum 96 M + strain 48 N + newest g written 24 N
                    + older g read (f_j - 1) 24 N (+ Lame 8 N shared)
       y (inverse + forward) : 2 x 96 M
-      x : 96 M per evaluation (setup, every check, every re-evaluation); with
+      x : 120 M per evaluation (setup, every check, every re-evaluation): the
+          spectrum read and written and the base divergence d_base read; with
           Anderson also the residual's three-complex-per-mode form y written 24 M
       aa_gram : Anderson Gram row, the f newest y read, f * 24 M,
           f = min(e + 1, depth + 1) at the e-th evaluation
@@ -50,7 +51,8 @@ def model_bytes(kind, N, M, iters, depth, nrebase=0):
     if kind == "y":
         return sum(2 * k for k in its) * 96 * M
     if kind == "x":
-        b = (sum(k + 1 for k in its) + nrebase * len(its)) * 96 * M
+        # spectrum in and out (96 M) and the base divergence d_base read (24 M)
+        b = (sum(k + 1 for k in its) + nrebase * len(its)) * 120 * M
         if depth > 0:
             b += sum(k + 1 for k in its) * 24 * M
         return b
@@ -256,7 +258,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--grid", type=int, default=256)
-    ap.add_argument("--depth", type=int, default=5)
+    ap.add_argument("--depth", type=int, default=6)
     ap.add_argument("--methods", default="plain,aa,aa_ws")
     ap.add_argument("--rtol", type=float, default=1e-6)
     ap.add_argument("--no-cpu-baseline", action="store_true")

@@ -131,7 +131,7 @@ namespace lsk {
 // INPL: the transforms overwrite the h / g lines in place (each thread holds
 // at most one line segment of stages 1 and 3 in registers across a barrier),
 // which halves the LDS of a block.
-template <int A, int B, int TM, int NT, bool REC, bool INPL>
+template <int A, int B, int TM, int NT, bool REC, int MODE>
 __global__ void __launch_bounds__(NT) k_xpass_h(GridDev g, const RhsDesc* __restrict__ rhs,
                                                 const RhsState* __restrict__ st, float2* __restrict__ spec,
                                                 const float2* __restrict__ dbase, double* __restrict__ partials,
@@ -139,6 +139,9 @@ __global__ void __launch_bounds__(NT) k_xpass_h(GridDev g, const RhsDesc* __rest
                                                 int do_gamma, AAParams aa) {
   constexpr int N = A * B;
   constexpr int LH = 3 * TM;  // h / g lines
+  constexpr bool INPL = MODE != 0;
+  constexpr bool SPLIT = MODE == 2;
+  static_assert(!SPLIT || swz_ok<N, B>(), "the split stage 2 runs on the swizzled layout");
   // in place: the transforms share the h / g lines' N LH complex, unpadded
   using L = std::conditional_t<INPL && swz_ok<N, B>(), SwzLayout<N, B>, RegLayout<N, B>>;
   static_assert(!INPL || B * LH <= NT, "in-place stages 1 and 3 need one item per thread");
@@ -160,7 +163,9 @@ __global__ void __launch_bounds__(NT) k_xpass_h(GridDev g, const RhsDesc* __rest
   const long long colmax = 6LL * nm;
   float2* S = spec + (long long)r * g.M * 6 + 6LL * m0;
   const float2* DB = dbase ? dbase + (long long)r * db_per_rhs + (long long)tile * N * TM * 3 : nullptr;
-  const float2* __restrict__ tw = g.px.tw;
+  // the x twiddles e^{-2 pi i j / N} in LDS (indices t k1 < N of stages 1 and 2)
+  __shared__ float2 tw[N];
+  for (int i = threadIdx.x; i < N; i += NT) tw[i] = g.px.tw[i];
   const RhsDesc d = rhs[r];
   if (threadIdx.x < TM) {
     const int gm = m0 + threadIdx.x;
@@ -175,31 +180,32 @@ __global__ void __launch_bounds__(NT) k_xpass_h(GridDev g, const RhsDesc* __rest
   }
   __syncthreads();
 
-  // stage 0: h_i(x) from the stress at x and x + 1 (the second row is read
-  // again by the neighbouring item and comes from the cache)
+  // stage 0: h_i(x) from the stress at x and x + 1.
+  // h_i = alpha (b_ix - a_ix) / 2i + (w_i(a) + w_i(b)) / 2, w_i = beta s_iy + gamma s_iz,
+  // a / b the stress at x / x + 1 (components 0 xx, 1 yy, 2 zz, 3 xy, 4 xz, 5 yz;
+  // i = x: (xx, xy, xz); i = y: (xy, yy, yz); i = z: (xz, yz, zz))
+  auto h_of = [&](int m, const float4& a01, const float4& a23, const float4& a45, const float4& b01,
+                  const float4& b23, const float4& b45, int x) {
+    const float ha = 0.5f * coef[m][0], be = coef[m][1], ga = coef[m][2];
+    const float wx0r = be * (a23.z + b23.z) + ga * (a45.x + b45.x), wx0i = be * (a23.w + b23.w) + ga * (a45.y + b45.y);
+    const float wx1r = be * (a01.z + b01.z) + ga * (a45.z + b45.z), wx1i = be * (a01.w + b01.w) + ga * (a45.w + b45.w);
+    const float wx2r = be * (a45.z + b45.z) + ga * (a23.x + b23.x), wx2i = be * (a45.w + b45.w) + ga * (a23.y + b23.y);
+    H[x * LH + 3 * m] = make_float2(ha * (b01.y - a01.y) + 0.5f * wx0r, -ha * (b01.x - a01.x) + 0.5f * wx0i);
+    H[x * LH + 3 * m + 1] = make_float2(ha * (b23.w - a23.w) + 0.5f * wx1r, -ha * (b23.z - a23.z) + 0.5f * wx1i);
+    H[x * LH + 3 * m + 2] = make_float2(ha * (b45.y - a45.y) + 0.5f * wx2r, -ha * (b45.x - a45.x) + 0.5f * wx2i);
+  };
+  // one item per (x, mode); the row x + 1 is read again by the neighbouring
+  // item and comes from the cache
   for (int it = threadIdx.x; it < N * TM; it += NT) {
     const int x = it / TM, m = it - x * TM;
-    float2 h0 = make_float2(0.f, 0.f), h1 = h0, h2 = h0;
     if (valid[m]) {
       const int x1 = (x + 1 == N) ? 0 : x + 1;
       const float2* p0 = S + x * colmax + 6 * m;
       const float2* p1 = S + x1 * colmax + 6 * m;
-      const float4 a01 = ld4(p0), a23 = ld4(p0 + 2), a45 = ld4(p0 + 4);
-      const float4 b01 = ld4(p1), b23 = ld4(p1 + 2), b45 = ld4(p1 + 4);
-      const float ha = 0.5f * coef[m][0], be = coef[m][1], ga = coef[m][2];
-      // sigma components at x (a) and x + 1 (b): 0 xx, 1 yy, 2 zz, 3 xy, 4 xz, 5 yz
-      // h_i = alpha (b_ix - a_ix) / 2i + (w_i(a) + w_i(b)) / 2, w_i = beta s_iy + gamma s_iz
-      // i = x: (xx, xy, xz); i = y: (xy, yy, yz); i = z: (xz, yz, zz)
-      const float wx0r = be * (a23.z + b23.z) + ga * (a45.x + b45.x), wx0i = be * (a23.w + b23.w) + ga * (a45.y + b45.y);
-      const float wx1r = be * (a01.z + b01.z) + ga * (a45.z + b45.z), wx1i = be * (a01.w + b01.w) + ga * (a45.w + b45.w);
-      const float wx2r = be * (a45.z + b45.z) + ga * (a23.x + b23.x), wx2i = be * (a45.w + b45.w) + ga * (a23.y + b23.y);
-      h0 = make_float2(ha * (b01.y - a01.y) + 0.5f * wx0r, -ha * (b01.x - a01.x) + 0.5f * wx0i);
-      h1 = make_float2(ha * (b23.w - a23.w) + 0.5f * wx1r, -ha * (b23.z - a23.z) + 0.5f * wx1i);
-      h2 = make_float2(ha * (b45.y - a45.y) + 0.5f * wx2r, -ha * (b45.x - a45.x) + 0.5f * wx2i);
+      h_of(m, ld4(p0), ld4(p0 + 2), ld4(p0 + 4), ld4(p1), ld4(p1 + 2), ld4(p1 + 4), x);
+    } else {
+      H[x * LH + 3 * m] = H[x * LH + 3 * m + 1] = H[x * LH + 3 * m + 2] = make_float2(0.f, 0.f);
     }
-    H[x * LH + 3 * m] = h0;
-    H[x * LH + 3 * m + 1] = h1;
-    H[x * LH + 3 * m + 2] = h2;
   }
   if (tile == 0 && threadIdx.x < 64) {
     // mean stress of the line kx = 0 of mode 0 (denominator of the residual):
@@ -256,23 +262,93 @@ __global__ void __launch_bounds__(NT) k_xpass_h(GridDev g, const RhsDesc* __rest
   const bool rec = REC && do_gamma;
   float2* Un = rec ? aa.U + ((long long)r * aa.m1 + aa.head[r]) * aa.u_per_rhs + (long long)tile * N * TM * 3
                    : nullptr;
-  for (int it = threadIdx.x; it < A * TM; it += NT) {
-    const int m = it / A, k1 = it - m * A;
-    if (!valid[m]) continue;
-    float2 u[3][B];
+  if constexpr (!SPLIT) {
+    for (int it = threadIdx.x; it < A * TM; it += NT) {
+      const int m = it / A, k1 = it - m * A;
+      if (!valid[m]) continue;
+      float2 u[3][B];
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
+      for (int i = 0; i < 3; ++i) {
 #pragma unroll
-      for (int t = 0; t < B; ++t) u[i][t] = W[L::at(3 * m + i, t + B * k1)];
-      dft_reg<B, false>(u[i]);
+        for (int t = 0; t < B; ++t) u[i][t] = W[L::at(3 * m + i, t + B * k1)];
+        dft_reg<B, false>(u[i]);
+      }
+      const float al = coef[m][0], be = coef[m][1], ga = coef[m][2];
+      double e2 = 0.0;
+#pragma unroll
+      for (int k2 = 0; k2 < B; ++k2) {
+        const int kx = k1 + A * k2;
+        float2 d0 = u[0][k2], d1 = u[1][k2], d2 = u[2][k2];
+        // tile entry (kx, m, i) at ((k2 3 + i) TM + m) A + k1 (DbLayout, three-line order)
+        const int eo = (k2 * 3 * TM + m) * A + k1;
+        if (DB) {
+          const float2 ph = g.phx[kx];
+          d0 = cadd(d0, cmul(ph, DB[eo]));
+          d1 = cadd(d1, cmul(ph, DB[eo + TM * A]));
+          d2 = cadd(d2, cmul(ph, DB[eo + 2 * TM * A]));
+        }
+        e2 += (double)d0.x * d0.x + (double)d0.y * d0.y + (double)d1.x * d1.x + (double)d1.y * d1.y +
+              (double)d2.x * d2.x + (double)d2.y * d2.y;
+        const float sxk = g.sx[kx], cxk = g.cx[kx];
+        const float k0 = al * sxk;
+        const float kk1 = be * cxk;
+        const float kk2 = ga * cxk;
+        const float q2 = k0 * k0 + kk1 * kk1 + kk2 * kk2;
+        if (q2 > 0.f) {
+          const float inv = 1.0f / sqrtf(q2);
+          const float n0 = k0 * inv, n1 = kk1 * inv, n2 = kk2 * inv;
+          const float rhox = (n0 * d0.x + n1 * d1.x + n2 * d2.x) * inv, rhoy = (n0 * d0.y + n1 * d1.y + n2 * d2.y) * inv;
+          const float cA = d.cA * inv * inv, cB = d.cB * inv;
+          u[0][k2] = make_float2(cA * d0.x + cB * rhox * n0, cA * d0.y + cB * rhoy * n0);
+          u[1][k2] = make_float2(cA * d1.x + cB * rhox * n1, cA * d1.y + cB * rhoy * n1);
+          u[2][k2] = make_float2(cA * d2.x + cB * rhox * n2, cA * d2.y + cB * rhoy * n2);
+          if (rec)
+            aa_y(coef[m][3], q2, inv, n0, n1, n2, u[0][k2], u[1][k2], u[2][k2], Un[eo], Un[eo + TM * A],
+                 Un[eo + 2 * TM * A]);
+        } else {
+          u[0][k2] = u[1][k2] = u[2][k2] = make_float2(0.f, 0.f);
+          if (rec) Un[eo] = Un[eo + TM * A] = Un[eo + 2 * TM * A] = make_float2(0.f, 0.f);
+        }
+      }
+      acc += (double)coef[m][3] * e2;
+      if (do_gamma) {
+        float2 twk[B];
+#pragma unroll
+        for (int t = 1; t < B; ++t) twk[t] = tw[t * k1];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          dft_reg<B, true>(u[i]);
+#pragma unroll
+          for (int t = 1; t < B; ++t) u[i][t] = cmulc(u[i][t], twk[t]);
+#pragma unroll
+          for (int t = 0; t < B; ++t) W[L::at(3 * m + i, t + B * k1)] = u[i][t];
+        }
+      }
     }
-    const float al = coef[m][0], be = coef[m][1], ga = coef[m][2];
-    double e2 = 0.0;
+  } else {
+    // the same work in three steps with one transform line (or one mode
+    // point) per item, so that few registers are live: (a) forward B-point
+    // DFTs in place, the output k2 of item (k1, line) at position k2 + B k1;
+    // (b) residual, d_base, Gamma0 and the Anderson record per (kx, mode);
+    // (c) inverse B-point DFTs and the conjugate twiddles, in place.
+    for (int it = threadIdx.x; it < A * LH; it += NT) {
+      const int line = it / A, k1 = it - line * A;
+      if (!valid[line / 3]) continue;
+      float2 u[B];
 #pragma unroll
-    for (int k2 = 0; k2 < B; ++k2) {
-      const int kx = k1 + A * k2;
-      float2 d0 = u[0][k2], d1 = u[1][k2], d2 = u[2][k2];
-      // tile entry (kx, m, i) at ((k2 3 + i) TM + m) A + k1 (DbLayout, three-line order)
+      for (int t = 0; t < B; ++t) u[t] = W[L::at(line, t + B * k1)];
+      dft_reg<B, false>(u);
+#pragma unroll
+      for (int k2 = 0; k2 < B; ++k2) W[L::at(line, k2 + B * k1)] = u[k2];
+    }
+    __syncthreads();
+    for (int it = threadIdx.x; it < N * TM; it += NT) {
+      const int m = it / N, kx = it - m * N;
+      if (!valid[m]) continue;
+      const int k2 = kx / A, k1 = kx - k2 * A;
+      const int p0 = L::at(3 * m, k2 + B * k1), p1 = L::at(3 * m + 1, k2 + B * k1),
+                p2 = L::at(3 * m + 2, k2 + B * k1);
+      float2 d0 = W[p0], d1 = W[p1], d2 = W[p2];
       const int eo = (k2 * 3 * TM + m) * A + k1;
       if (DB) {
         const float2 ph = g.phx[kx];
@@ -280,41 +356,43 @@ __global__ void __launch_bounds__(NT) k_xpass_h(GridDev g, const RhsDesc* __rest
         d1 = cadd(d1, cmul(ph, DB[eo + TM * A]));
         d2 = cadd(d2, cmul(ph, DB[eo + 2 * TM * A]));
       }
-      e2 += (double)d0.x * d0.x + (double)d0.y * d0.y + (double)d1.x * d1.x + (double)d1.y * d1.y +
-            (double)d2.x * d2.x + (double)d2.y * d2.y;
-      const float sxk = g.sx[kx], cxk = g.cx[kx];
-      const float k0 = al * sxk;
-      const float kk1 = be * cxk;
-      const float kk2 = ga * cxk;
+      acc += (double)coef[m][3] * ((double)d0.x * d0.x + (double)d0.y * d0.y + (double)d1.x * d1.x +
+                                   (double)d1.y * d1.y + (double)d2.x * d2.x + (double)d2.y * d2.y);
+      if (!do_gamma) continue;
+      const float k0 = coef[m][0] * g.sx[kx];
+      const float kk1 = coef[m][1] * g.cx[kx];
+      const float kk2 = coef[m][2] * g.cx[kx];
       const float q2 = k0 * k0 + kk1 * kk1 + kk2 * kk2;
+      float2 u0 = make_float2(0.f, 0.f), u1 = u0, u2 = u0;
       if (q2 > 0.f) {
         const float inv = 1.0f / sqrtf(q2);
         const float n0 = k0 * inv, n1 = kk1 * inv, n2 = kk2 * inv;
         const float rhox = (n0 * d0.x + n1 * d1.x + n2 * d2.x) * inv, rhoy = (n0 * d0.y + n1 * d1.y + n2 * d2.y) * inv;
         const float cA = d.cA * inv * inv, cB = d.cB * inv;
-        u[0][k2] = make_float2(cA * d0.x + cB * rhox * n0, cA * d0.y + cB * rhoy * n0);
-        u[1][k2] = make_float2(cA * d1.x + cB * rhox * n1, cA * d1.y + cB * rhoy * n1);
-        u[2][k2] = make_float2(cA * d2.x + cB * rhox * n2, cA * d2.y + cB * rhoy * n2);
-        if (rec)
-          aa_y(coef[m][3], q2, inv, n0, n1, n2, u[0][k2], u[1][k2], u[2][k2], Un[eo], Un[eo + TM * A],
-               Un[eo + 2 * TM * A]);
-      } else {
-        u[0][k2] = u[1][k2] = u[2][k2] = make_float2(0.f, 0.f);
-        if (rec) Un[eo] = Un[eo + TM * A] = Un[eo + 2 * TM * A] = make_float2(0.f, 0.f);
+        u0 = make_float2(cA * d0.x + cB * rhox * n0, cA * d0.y + cB * rhoy * n0);
+        u1 = make_float2(cA * d1.x + cB * rhox * n1, cA * d1.y + cB * rhoy * n1);
+        u2 = make_float2(cA * d2.x + cB * rhox * n2, cA * d2.y + cB * rhoy * n2);
+        if (rec) aa_y(coef[m][3], q2, inv, n0, n1, n2, u0, u1, u2, Un[eo], Un[eo + TM * A], Un[eo + 2 * TM * A]);
+      } else if (rec) {
+        Un[eo] = Un[eo + TM * A] = Un[eo + 2 * TM * A] = make_float2(0.f, 0.f);
       }
+      W[p0] = u0;
+      W[p1] = u1;
+      W[p2] = u2;
     }
-    acc += (double)coef[m][3] * e2;
     if (do_gamma) {
-      float2 twk[B];
+      __syncthreads();
+      for (int it = threadIdx.x; it < A * LH; it += NT) {
+        const int line = it / A, k1 = it - line * A;
+        if (!valid[line / 3]) continue;
+        float2 u[B];
 #pragma unroll
-      for (int t = 1; t < B; ++t) twk[t] = tw[t * k1];
+        for (int k2 = 0; k2 < B; ++k2) u[k2] = W[L::at(line, k2 + B * k1)];
+        dft_reg<B, true>(u);
 #pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        dft_reg<B, true>(u[i]);
+        for (int t = 1; t < B; ++t) u[t] = cmulc(u[t], tw[t * k1]);
 #pragma unroll
-        for (int t = 1; t < B; ++t) u[i][t] = cmulc(u[i][t], twk[t]);
-#pragma unroll
-        for (int t = 0; t < B; ++t) W[L::at(3 * m + i, t + B * k1)] = u[i][t];
+        for (int t = 0; t < B; ++t) W[L::at(line, t + B * k1)] = u[t];
       }
     }
   }

@@ -129,11 +129,13 @@ struct Tile {
   F(96, 12, 8, 32, 256) F(96, 12, 8, 64, 512) F(64, 8, 8, 32, 256) F(64, 8, 8, 64, 512)         \
   F(48, 6, 8, 32, 256) F(48, 6, 8, 64, 512) F(32, 8, 4, 64, 256) F(32, 8, 4, 128, 512)          \
   F(160, 5, 32, 32, 256) F(160, 32, 5, 32, 160) F(160, 32, 5, 64, 320) F(160, 5, 32, 64, 320)
-// x pass with three-line transforms (k_xpass_h): (n, A, B, TM, NT, in place)
+// x pass with three-line transforms (k_xpass_h): (n, A, B, TM, NT, mode); mode 0: padded
+// LDS, 1: transforms in place, 2: in place with the split stage 2
 #define LS_REG_H_LIST(F)                                                                                     \
-  F(256, 32, 8, 8, 256, 1) F(256, 32, 8, 4, 256, 0) F(256, 16, 16, 4, 256, 0) F(256, 32, 8, 4, 128, 0)      \
-  F(256, 32, 8, 4, 128, 1) F(128, 16, 8, 8, 256, 0) F(128, 16, 8, 4, 256, 0) F(128, 16, 8, 8, 256, 1)        \
-  F(96, 12, 8, 8, 256, 0) F(96, 12, 8, 4, 128, 0) F(96, 12, 8, 8, 256, 1) F(64, 8, 8, 8, 256, 0)             \
+  F(256, 16, 16, 8, 512, 2) F(256, 32, 8, 8, 256, 1) F(256, 32, 8, 4, 256, 0) F(256, 16, 16, 4, 256, 0)      \
+  F(256, 32, 8, 4, 128, 0) F(256, 32, 8, 4, 128, 1) F(256, 16, 16, 4, 256, 2) F(128, 16, 8, 8, 256, 0)      \
+  F(128, 16, 8, 4, 256, 0) F(128, 16, 8, 8, 256, 1) F(96, 12, 8, 8, 256, 0) F(96, 12, 8, 4, 128, 0)          \
+  F(96, 12, 8, 8, 256, 1) F(64, 8, 8, 8, 256, 0)                                                             \
   F(64, 8, 8, 16, 512, 0) F(64, 8, 8, 16, 512, 1) F(48, 6, 8, 16, 256, 0) F(48, 6, 8, 8, 256, 0)             \
   F(32, 8, 4, 16, 256, 0) F(32, 8, 4, 32, 512, 0) F(160, 32, 5, 8, 256, 1) F(160, 32, 5, 8, 256, 0)          \
   F(160, 32, 5, 16, 512, 1) F(160, 32, 5, 4, 128, 0)
