@@ -8,7 +8,7 @@ in [0.3, 0.6], fibre/matrix shear-modulus ratio log-uniform in [5, 100];
 matrix mu = 1.111, nu = 0.35; fibre nu = 0.2; six unit load cases each.
 
 Samples are grouped by grid and solved in batches (all load cases of up to
-`max_microstructures` microstructures per solver call, Anderson depth 4 by
+`max_microstructures` microstructures per solver call, Anderson depth 6 by
 default), with the tiles of each grid picked by the solver's autotuner and the
 iterations between host synchronisations replayed as captured HIP graphs.
 This replaces the reference's one-call-per-load-case loop
@@ -88,7 +88,7 @@ def _batch_size(n, depth, budget_bytes):
     return max(1, int(budget_bytes // (6 * per_rhs)))
 
 
-def solve_batch(params, depth=4, rtol=1e-6, maxiter=3000, graph=True, autotune=True,
+def solve_batch(params, depth=6, rtol=1e-6, maxiter=3000, graph=True, autotune=True,
                 budget_bytes=48 << 30, max_microstructures=128, device="cuda", log=None):
     """Effective tensors of the given samples.  Returns a dict with
     ceff [n][6][6] (row k = <sigma> under eps_bar = e_k), iters [n][6] and
@@ -144,7 +144,7 @@ def solve_batch(params, depth=4, rtol=1e-6, maxiter=3000, graph=True, autotune=T
 def main(argv=None):
     ap = argparse.ArgumentParser(description="C_eff of an uneven batch of seeded cross-ply microstructures")
     ap.add_argument("--count", type=int, default=384)
-    ap.add_argument("--depth", type=int, default=4)
+    ap.add_argument("--depth", type=int, default=6)
     ap.add_argument("--rtol", type=float, default=1e-6)
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
