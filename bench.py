@@ -252,6 +252,103 @@ def dataset_cpu_baseline(gs, gen, mean_iters):
             "rasterise_s": t_raster, "s_per_iteration": s_it, "extrapolated": True}
 
 
+PHASEFIELD_METRIC = ("phase-field staggered load steps/sec: the reference's phase-field script "
+                     "(55x50x1 notched plate, 51 load steps, AA(3) elasticity to 1e-6, FP64 phase field to 1e-9)")
+
+
+def _notched_plate(gs, lam_solid=121.15, mu_solid=80.77, margin=0.05, crack_length=0.5):
+    """Lame fields of the reference's phase-field script (tests/phasefield.py:
+    45-131 with shape="homogeneous"): a homogeneous plate, zero stiffness in the
+    margins x < margin and x > Lx - margin, and a one-voxel crack from the
+    left margin along y = Ly / 2.  Returns float32 (lmbda, mu)."""
+    import numpy as np
+
+    h = [gs.Lx / gs.nx, gs.Ly / gs.ny, gs.Lz / gs.nz]
+    X, Y, _ = np.meshgrid(*[hh * (0.5 + np.arange(n)) for n, hh in zip((gs.nx, gs.ny, gs.nz), h)], indexing="ij")
+    lam = np.full(X.shape, lam_solid)
+    mu = np.full(X.shape, mu_solid)
+    margin_bound = (X < margin) | (X > gs.Lx - margin)
+    crack = (X <= margin + crack_length) & (Y > gs.Ly / 2 - h[1]) & (Y < gs.Ly / 2)
+    lam[margin_bound | crack] = 0.0
+    mu[margin_bound | crack] = 0.0
+    return lam.astype(np.float32), mu.astype(np.float32)
+
+
+def phasefield_main(args):
+    """One step = one staggered load step of the reference's phase-field script
+    (tests/phasefield.py:160-225): phase field (FP64 fixed point to 1e-9), the
+    damaged elasticity solve (Anderson depth 3 to rtol 1e-6), the history
+    update; eps_yy from 0 to 0.01 in 51 steps."""
+    import numpy as np
+    import torch
+
+    from jaxmaterials_amd.common import GridSpec
+    from jaxmaterials_amd.solver.phasefield import staggered_steps
+
+    gs = GridSpec(55, 50, 1, 1.0, 1.0, 0.015)
+    lam, mu = _notched_plate(gs)
+    ebars = [np.array([0.0, e, 0.0, 0.0, 0.0, 0.0]) for e in np.linspace(0, 0.01, 51)]
+    kw = dict(gc=2.7e-3, lc=0.015, k_stab=1e-6, depth=3, rtol=1e-6, maxiter=1000, pf_tolerance=1e-9, pf_maxiter=10000)
+    lt, mt = torch.from_numpy(lam).cuda(), torch.from_numpy(mu).cuda()
+    for _ in range(args.warmup):
+        staggered_steps(lt, mt, ebars[:3], gs, **kw)
+    times, out = [], None
+    for _ in range(args.steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = staggered_steps(lt, mt, ebars, gs, **kw)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    sec = float(np.mean(times))
+    its = [st["elasticity_iterations"] for st in out["steps"]]
+    pfits = [st["pf_iterations"] for st in out["steps"]]
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import phasefield_oracle as PF
+
+        t0 = time.perf_counter()
+        d = np.zeros(lam.shape)
+        H = np.zeros(lam.shape)
+        sig_o = None
+        for eb in ebars:
+            d, _ = PF.phase_field_solve(H, d, kw["gc"], kw["lc"], gs, tolerance=1e-9, maxiter=10000)
+            if np.any(eb):
+                eps, sig_o, _ = PF.elasticity_solve(lam, mu, eb, d.astype(np.float32), kw["k_stab"], gs, rtol=1e-6,
+                                                    depth=3, maxiter=1000)
+                H = np.maximum(H, PF.strain_energy_plus(lam, mu, eps))
+        t_cpu = time.perf_counter() - t0
+        cpu = {"value": len(ebars) / t_cpu, "unit": "steps/s", "cores": 1, "kind": "port",
+               "sample": f"the whole 51-step script on the FP64 NumPy restatement, {t_cpu:.1f} s (NumPy FFTs, "
+                         f"one thread)",
+               "final_sigma_yy": float(sig_o.mean(axis=(1, 2, 3))[1]) if sig_o is not None else None}
+    line = {
+        "metric": PHASEFIELD_METRIC,
+        "value": len(ebars) / sec,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sec * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 elasticity, fp64 phase field",
+        "data": "synthetic",
+        "config": {"workload": "reference phase-field script: 55x50x1 plate of 1 x 1 x 0.015, lambda 121.15, "
+                               "mu 80.77, zero stiffness in 0.05 margins and a 0.5 crack, eps_yy 0 -> 0.01 in 51 "
+                               "steps, gc 2.7e-3, lc 0.015, k 1e-6",
+                   "grid": [55, 50, 1], "load_steps": len(ebars)},
+        "script_seconds": sec,
+        "elasticity_iterations_total": int(sum(its)),
+        "phase_field_iterations_total": int(sum(pfits)),
+        "final_sigma_yy": out["steps"][-1]["sigma_mean"][1],
+        "max_damage": float(out["damage"].max()),
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -262,9 +359,9 @@ def main():
     ap.add_argument("--methods", default="plain,aa,aa_ws")
     ap.add_argument("--rtol", type=float, default=1e-6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=("crossply256", "dataset"), default="crossply256",
+    ap.add_argument("--workload", choices=("crossply256", "dataset", "phasefield"), default="crossply256",
                     help="crossply256: the C_eff solve benchmark (default); dataset: the reference's "
-                         "layered-fibre dataset generation")
+                         "layered-fibre dataset generation; phasefield: the reference's phase-field script")
     ap.add_argument("--samples", type=int, default=128, help="dataset workload: samples per step")
     ap.add_argument("--batch", type=int, default=32, help="dataset workload: samples per solver call")
     args = ap.parse_args()
@@ -274,6 +371,9 @@ def main():
         sys.exit(2)
     if args.workload == "dataset":
         dataset_main(args)
+        return
+    if args.workload == "phasefield":
+        phasefield_main(args)
         return
 
     os.environ["LS_TIMING"] = "1"

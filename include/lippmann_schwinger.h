@@ -154,6 +154,47 @@ int ls_rasterise_layered_fibres(const int *voxels, const double *extents,
                                 const double *lambda3, int dtype_bytes,
                                 void *d_out, void *stream);
 
+/* ---- phase-field fracture (src/jaxmaterials/solver/lippmann_schwinger.py:36-100, 334-547) */
+
+/* Material law of the handle's next solves: the damaged law of
+ * compute_sigma_damaged (lippmann_schwinger.py:36-74),
+ * sigma = g sigma+ + sigma-, g = (1 - d)^2 + k, with the tension part from
+ * the eigen-split of the strain matrix as the reference forms it; d_damage =
+ * device [nmat][N] damage fields (float32), or NULL for Hooke's law.  With
+ * it, ls_solver_solve performs elasticity_solve (lippmann_schwinger.py:
+ * 334-462): the same iteration, stop test and Anderson mixing, the reference
+ * medium still from lambda and mu.  The field must stay valid while set.
+ * 0 on success. */
+int ls_solver_set_damage(ls_solver *h, const float *d_damage, float k);
+
+/* phase_field_solve (lippmann_schwinger.py:465-547) on the handle's grid, in
+ * FP64: the fixed point d = IFFT[FFT(chi) / (A0 + |xi~|^2)],
+ * chi = B - (A - A0) d, A = 1/lc^2 + 2H/(gc lc), B = 2H/(gc lc),
+ * A0 = (min A + max A) / 2, from chi = B - (A - A0) d_old until
+ * ||chi_new - chi|| / ||chi_new|| <= tol or maxiter iterations.  Device
+ * FP64 fields d_H (history), d_dold, d_out [N]; the iteration count and the
+ * last residual go to *iters and *residual.  Every axis <= 3840.  0 on
+ * success. */
+int ls_solver_phase_field(ls_solver *h, const double *d_H, const double *d_dold,
+                          double gc, double lc, double tol, int maxiter,
+                          double *d_out, int *iters, double *residual);
+
+/* compute_sigma_damaged (lippmann_schwinger.py:36-74) pointwise on device
+ * arrays: d_eps, d_sigma [6][N] float32, d_lambda, d_mu [N], d_damage [N]
+ * or NULL (undamaged: g = 1).  Synchronous on `stream`. */
+int ls_damaged_stress(const int *voxels, const float *d_lambda,
+                      const float *d_mu, const float *d_eps,
+                      const float *d_damage, float k, float *d_sigma,
+                      void *stream);
+
+/* compute_strain_energy (lippmann_schwinger.py:77-100): psi+ per voxel into
+ * d_psi (FP64 [N], may be NULL) and, if d_history is not NULL, the history
+ * update H = max(H, psi+) of the staggered scheme (tests/phasefield.py:
+ * 221-223).  Synchronous on `stream`. */
+int ls_strain_energy(const int *voxels, const float *d_lambda, const float *d_mu,
+                     const float *d_eps, double *d_psi, double *d_history,
+                     void *stream);
+
 /* Measured HBM streaming ceilings of the current device (a measurement aid,
  * no counterpart in the reference): grid-stride float4 kernels over
  * `bytes`-sized arrays (three are allocated), best of `reps` runs each;

@@ -58,6 +58,23 @@ SIGNATURES = {
          ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p],
     ),
     "ls_hbm_stream_probe": (ctypes.c_int, [ctypes.c_longlong, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
+    # phase-field fracture
+    "ls_solver_set_damage": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float]),
+    "ls_solver_phase_field": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+         ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+    ),
+    "ls_damaged_stress": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float,
+         ctypes.c_void_p, ctypes.c_void_p],
+    ),
+    "ls_strain_energy": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+         ctypes.c_void_p],
+    ),
     # reference fibre movement (include/fibres.h)
     "dist_periodic": (
         None,

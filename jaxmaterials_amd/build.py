@@ -17,7 +17,8 @@ LIB_NAME = "liblippmannschwinger.so"
 LIB_PATH = os.path.join(HERE, LIB_NAME)
 
 # every HIP source of the library; each compiles to its own object in parallel
-SOURCES = ["ls_solver.hip", "microstructure.hip", "conv3d.hip", "derivatives.hip", "stream_probe.hip"]
+SOURCES = ["ls_solver.hip", "microstructure.hip", "conv3d.hip", "derivatives.hip", "stream_probe.hip",
+           "phasefield.hip"]
 # host-only C++ sources, compiled by g++ (the reference fibre movement is
 # compiled by g++ in the reference too, and its results depend on the
 # standard library's random engines and distributions)

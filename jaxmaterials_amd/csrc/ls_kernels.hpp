@@ -36,6 +36,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "damage.hpp"
 #include "fft_lds.hpp"
 
 namespace lsk {
@@ -57,6 +58,11 @@ struct GridDev {
   AxisPlan px, py, pz;
   FastDiv f_nz, f_P;  // divisions by nz and P
   int lsz;            // padded LDS line stride of the z-line kernels
+  // phase-field damage law (null dmg: Hooke's law): per-material damage
+  // fields [nmat][N], the stability constant k, and the load cases' bases
+  const float* dmg;
+  float kdmg;
+  const float* base;
 };
 
 // Layout of d_base matched to the tiles of the FP32 x pass: [tile][x][m][3]
@@ -236,7 +242,7 @@ __device__ __forceinline__ void store_half_pairs(const V* lds, const GridDev& g,
 // ---------------------------------------------------------------------------
 // z-line pass.  Block = Lz consecutive (x,y) lines of one load case.
 // LDS: Z[l][p][k], p = component pair (0,1),(2,3),(4,5): z_p = s_{2p} + i s_{2p+1}.
-template <int MODE, int E, int NT>
+template <int MODE, int E, int NT, bool DMG = false>
 __global__ void __launch_bounds__(NT) k_zline(GridDev g, const RhsDesc* __restrict__ rhs,
                                               RhsState* __restrict__ st, const float* __restrict__ lam,
                                               const float* __restrict__ mu, float* __restrict__ eps,
@@ -354,6 +360,23 @@ __global__ void __launch_bounds__(NT) k_zline(GridDev g, const RhsDesc* __restri
           hooke(de, d.lam0, d.mu0, s);
         } else {
           hooke(ev, LA[v], MU[v], s);
+          if constexpr (DMG) {
+            // damaged law: sigma(base + delta) - sigma(base)
+            //   = C delta + (g - 1) (sigma+(base + delta) - sigma+(base)), in FP64
+            const float* B_ = g.base + (long long)r * 6 * N;
+            double b[6], t[6], sp1[6], sp0[6];
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {
+              b[c] = B_[c * N + v];
+              t[c] = b[c] + (double)ev[c];
+            }
+            const double lv = LA[v], mv = MU[v];
+            sigma_tension(t, lv, mv, sp1);
+            sigma_tension(b, lv, mv, sp0);
+            const double gm1 = degradation(g.dmg[(long long)d.mat * N + v], g.kdmg) - 1.0;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) s[c] += (float)(gm1 * (sp1[c] - sp0[c]));
+          }
         }
 #pragma unroll
         for (int p = 0; p < 3; ++p) lds[lds_idx<true>(l * 3 + p, j, 1, ls)] = make_float2(s[2 * p], s[2 * p + 1]);
@@ -585,6 +608,13 @@ __global__ void __launch_bounds__(NT) k_base_z(GridDev g, const RhsDesc* __restr
 #pragma unroll
     for (int c = 0; c < 6; ++c) ev[c] = B_[c * N + v];
     hooke(ev, (double)LA[v], (double)MU[v], s);
+    if (g.dmg) {
+      double sp[6];
+      sigma_tension(ev, (double)LA[v], (double)MU[v], sp);
+      const double gm1 = degradation(g.dmg[(long long)d.mat * N + v], g.kdmg) - 1.0;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) s[c] += gm1 * sp[c];
+    }
     if (T_) {
 #pragma unroll
       for (int c = 0; c < 6; ++c) s[c] += T_[c * N + v];
@@ -763,6 +793,15 @@ __global__ void __launch_bounds__(kThreads) k_stress(GridDev g, const RhsDesc* _
       for (int c = 0; c < 6; ++c) E_[c * N + v] = e[c];
     }
     hooke(e, LA[v], MU[v], s);
+    if (g.dmg) {
+      double ed[6], sp[6];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) ed[c] = e[c];
+      sigma_tension(ed, (double)LA[v], (double)MU[v], sp);
+      const double gm1 = degradation(g.dmg[(long long)d.mat * N + v], g.kdmg) - 1.0;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) s[c] += (float)(gm1 * sp[c]);
+    }
     if (T_) {
 #pragma unroll
       for (int c = 0; c < 6; ++c) s[c] += T_[c * N + v];

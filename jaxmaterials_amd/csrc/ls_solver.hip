@@ -22,6 +22,7 @@
 #include "../../include/lippmann_schwinger.h"
 #include "ls_kernels.hpp"
 #include "ls_passes_reg.hpp"
+#include "phasefield.hpp"
 
 using namespace lsk;
 
@@ -361,9 +362,15 @@ void zline(ls_solver* s, const float* lam, const float* mu, float* eps) {
   const AAParams aa = aa_params(s, 0);
   const unsigned grid = (unsigned)s->nblk_z * s->nrhs;
   const size_t lds = s->tz.lds;
-#define LS_Z(EV, NTV)                                                                                           \
-  k_zline<MODE, EV, NTV><<<grid, NTV, lds, s->stream>>>(s->gd, s->d_rhs, s->d_st, lam, mu, eps, s->spec, s->tz.lines, \
-                                                        s->nblk_z, aa)
+  // the damaged law is compiled for the two update modes only
+  constexpr bool kDmg = MODE == Z_UPD || MODE == Z_AA;
+#define LS_Z(EV, NTV)                                                                                            \
+  if (kDmg && s->gd.dmg)                                                                                         \
+    k_zline<MODE, EV, NTV, kDmg><<<grid, NTV, lds, s->stream>>>(s->gd, s->d_rhs, s->d_st, lam, mu, eps, s->spec, \
+                                                                s->tz.lines, s->nblk_z, aa);                     \
+  else                                                                                                           \
+    k_zline<MODE, EV, NTV, false><<<grid, NTV, lds, s->stream>>>(s->gd, s->d_rhs, s->d_st, lam, mu, eps, s->spec, \
+                                                                 s->tz.lines, s->nblk_z, aa)
   LS_DISPATCH32(s->tz, LS_Z)
 #undef LS_Z
   mark(s, MODE == Z_AA ? 4 : 0);
@@ -666,6 +673,9 @@ int create_impl(ls_solver* s, const int* voxels, const float* extents, int nrhs,
       dalloc(&s->partials, (size_t)nrhs * s->ny * s->P) || dalloc(&s->dc, (size_t)nrhs * 6) ||
       dalloc(&s->d_rhs, nrhs) || dalloc(&s->d_st, nrhs) || dalloc(&s->d_sel, nrhs))
     return -1;
+  g.base = s->base;
+  g.dmg = nullptr;  // Hooke's law until ls_solver_set_damage
+  g.kdmg = 0.f;
   LS_HIP(hipHostMalloc((void**)&s->h_st, nrhs * sizeof(RhsState), hipHostMallocDefault));
   s->chunk_stress = 8192;
   s->nblk_stress = (int)((s->N + s->chunk_stress - 1) / s->chunk_stress);
@@ -1195,6 +1205,126 @@ ls_solver* ls_solver_create(const int* voxels, const float* extents, int nrhs, i
 }
 
 void ls_solver_destroy(ls_solver* s) { free_solver(s); }
+
+int ls_solver_phase_field(ls_solver* s, const double* d_H, const double* d_dold, double gc, double lc, double tol,
+                          int maxiter, double* d_out, int* iters, double* residual) {
+  if (!s || !d_H || !d_dold || !d_out || !(gc > 0.0) || !(lc > 0.0) || maxiter < 0) {
+    fprintf(stderr, "liblippmannschwinger: ls_solver_phase_field: invalid arguments\n");
+    return -1;
+  }
+  const int nx = s->nx, ny = s->ny, nz = s->nz;
+  if (nx > kPfItems || ny > kPfItems || nz > kPfItems) {
+    fprintf(stderr, "liblippmannschwinger: ls_solver_phase_field: axis longer than %d\n", kPfItems);
+    return -1;
+  }
+  const long long N = s->N;
+  hipStream_t st = s->stream;
+  // tiles of the five passes (lines x length <= kPfItems per block)
+  const int L = std::max(1, std::min(1024, kPfItems / nz));
+  const int nblk_z = (nx * ny + L - 1) / L;
+  const size_t lds_z = (size_t)L * s->gd.lsz * sizeof(double2);
+  const int Ty = std::max(1, std::min(nz, kPfItems / ny));
+  const int nch_y = (nz + Ty - 1) / Ty;
+  const size_t lds_y = (size_t)ny * Ty * sizeof(double2);
+  const long long Q = (long long)ny * nz;
+  const int Tx = (int)std::max(1LL, std::min(Q, (long long)(kPfItems / nx)));
+  const long long nblk_x = (Q + Tx - 1) / Tx;
+  const size_t lds_x = (size_t)nx * Tx * sizeof(double2);
+  const int nb_mm = (int)std::min<long long>(1024, (N + kThreads - 1) / kThreads);
+  // FP64 symbol tables: sin / cos (pi k / n) with the exact zeros, 2 n / L
+  std::vector<double> tab(2 * (size_t)(nx + ny + nz));
+  const int ns[3] = {nx, ny, nz};
+  size_t off[3];
+  {
+    size_t o = 0;
+    for (int a = 0; a < 3; ++a) {
+      off[a] = o;
+      const int n = ns[a];
+      for (int k = 0; k < n; ++k) {
+        tab[o + k] = k == 0 ? 0.0 : std::sin(M_PI * k / n);
+        tab[o + n + k] = (n % 2 == 0 && 2 * k == n) ? 0.0 : std::cos(M_PI * k / n);
+      }
+      o += 2 * (size_t)n;
+    }
+  }
+  double2* C = nullptr;
+  double *chi = nullptr, *part = nullptr, *sums = nullptr, *dtab = nullptr;
+  int rc = -1;
+  do {
+    if (dalloc(&C, (size_t)N) || dalloc(&chi, (size_t)N) ||
+        dalloc(&part, (size_t)2 * std::max<long long>(nblk_z, nb_mm)) || dalloc(&sums, 2) ||
+        dalloc(&dtab, tab.size()))
+      break;
+    if (hipMemcpyAsync(dtab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess) break;
+    PfTables tb;
+    tb.sx = dtab + off[0];
+    tb.cx = dtab + off[0] + nx;
+    tb.sy = dtab + off[1];
+    tb.cy = dtab + off[1] + ny;
+    tb.sz = dtab + off[2];
+    tb.cz = dtab + off[2] + nz;
+    tb.gx = 2.0 * nx / (double)s->Lx;
+    tb.gy = 2.0 * ny / (double)s->Ly;
+    tb.gz = 2.0 * nz / (double)s->Lz;
+    // A0 from min / max of H (A increases with H)
+    k_pf_minmax<<<nb_mm, kThreads, 0, st>>>(d_H, N, part);
+    std::vector<double> mm(2 * (size_t)nb_mm);
+    if (hipMemcpyAsync(mm.data(), part, mm.size() * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      break;
+    double hmin = INFINITY, hmax = -INFINITY;
+    for (int b = 0; b < nb_mm; ++b) {
+      hmin = std::min(hmin, mm[2 * b]);
+      hmax = std::max(hmax, mm[2 * b + 1]);
+    }
+    const double gclc = gc * lc, inv_lc2 = 1.0 / (lc * lc);
+    const double A0 = 0.5 * ((inv_lc2 + 2.0 * hmin / gclc) + (inv_lc2 + 2.0 * hmax / gclc));
+    k_pf_init<<<nb_mm, kThreads, 0, st>>>(N, d_H, d_dold, inv_lc2, gclc, A0, chi, d_out);
+    const FastDiv fy = make_fastdiv(Ty), fx = make_fastdiv(Tx);
+    double res = 1.0;
+    int it = 0;
+    bool ok = hipGetLastError() == hipSuccess;
+    while (ok && res > tol && it < maxiter) {
+      k_pf_z_fwd<<<nblk_z, kPfNT, lds_z, st>>>(s->gd, chi, C, L);
+      k_pf_y<false><<<nx * nch_y, kPfNT, lds_y, st>>>(s->gd, C, fy, nch_y);
+      k_pf_x<<<(unsigned)nblk_x, kPfNT, lds_x, st>>>(s->gd, tb, C, fx, A0);
+      k_pf_y<true><<<nx * nch_y, kPfNT, lds_y, st>>>(s->gd, C, fy, nch_y);
+      k_pf_z_inv<<<nblk_z, kPfNT, lds_z, st>>>(s->gd, C, d_H, inv_lc2, gclc, A0, chi, d_out, part, L);
+      k_pf_sum<<<1, kThreads, 0, st>>>(part, nblk_z, sums);
+      double h[2];
+      ok = hipGetLastError() == hipSuccess &&
+           hipMemcpyAsync(h, sums, sizeof(h), hipMemcpyDeviceToHost, st) == hipSuccess &&
+           hipStreamSynchronize(st) == hipSuccess;
+      if (!ok) break;
+      const double nd = std::sqrt(h[0]), nc = std::sqrt(h[1]);
+      res = nc > 1e-12 ? nd / nc : 0.0;
+      ++it;
+    }
+    if (!ok) break;
+    if (iters) *iters = it;
+    if (residual) *residual = res;
+    rc = 0;
+  } while (0);
+  if (C) (void)hipFree(C);
+  if (chi) (void)hipFree(chi);
+  if (part) (void)hipFree(part);
+  if (sums) (void)hipFree(sums);
+  if (dtab) (void)hipFree(dtab);
+  if (rc != 0) fprintf(stderr, "liblippmannschwinger: ls_solver_phase_field failed\n");
+  return rc;
+}
+
+int ls_solver_set_damage(ls_solver* s, const float* d_damage, float k) {
+  if (!s) return -1;
+  s->gd.dmg = d_damage;
+  s->gd.kdmg = k;
+  if (s->gexec) {
+    // a captured iteration bakes in the material law
+    (void)hipGraphExecDestroy(s->gexec);
+    s->gexec = nullptr;
+  }
+  return 0;
+}
 
 int ls_solver_solve(ls_solver* s, const float* d_lambda, const float* d_mu, const int* rhs_mat,
                     const float* eps_bar, const float* d_tau, float* d_eps, float* d_sigma, int warm_start,

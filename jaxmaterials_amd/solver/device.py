@@ -118,6 +118,40 @@ class LSSolver:
         return {"epsilon": epsilon, "sigma": sigma, "iters": iters, "residual": res, "sigma_mean": smean,
                 "converged": rc == 0}
 
+    def set_damage(self, damage=None, k=1e-12):
+        """Material law of the following solves: the phase-field damaged law
+        (``ls_solver_set_damage``) with GPU float32 damage fields [nmat, nx, ny, nz]
+        (or [nx, ny, nz] when nmat == 1) and stability constant k, or Hooke's
+        law when damage is None.  The tensor is kept referenced while set."""
+        if damage is None:
+            self._damage = None
+            rc = self.lib.ls_solver_set_damage(self._h, None, 0.0)
+        else:
+            dm = damage.reshape((self.nmat,) + self.shape) if damage.dim() == 3 else damage
+            _check_dev("damage", dm, (self.nmat,) + self.shape)
+            self._damage = dm
+            rc = self.lib.ls_solver_set_damage(self._h, _ptr(dm), float(k))
+        if rc != 0:
+            raise RuntimeError("ls_solver_set_damage failed")
+
+    def phase_field(self, history, d_old, gc, lc, tolerance=1e-6, maxiter=1000, out=None):
+        """phase_field_solve on this solver's grid (``ls_solver_phase_field``):
+        GPU float64 [nx, ny, nz] history and previous damage; returns
+        (damage float64 tensor, iterations, last residual)."""
+        for name, t in (("history", history), ("d_old", d_old)):
+            if not t.is_cuda or t.dtype != torch.float64 or tuple(t.shape) != self.shape or not t.is_contiguous():
+                raise ValueError(f"{name} must be a contiguous float64 GPU tensor {self.shape}")
+        if out is None:
+            out = torch.empty_like(d_old)
+        it = ctypes.c_int(0)
+        res = ctypes.c_double(0.0)
+        torch.cuda.current_stream(history.device).synchronize()
+        rc = self.lib.ls_solver_phase_field(self._h, _ptr(history), _ptr(d_old), float(gc), float(lc), float(tolerance),
+                                            int(maxiter), _ptr(out), ctypes.byref(it), ctypes.byref(res))
+        if rc != 0:
+            raise RuntimeError("ls_solver_phase_field failed (see stderr)")
+        return out, it.value, res.value
+
     def autotune(self, lmbda, mu, epsilon=None, reps=3):
         """Time the candidate FFT tiles of the three passes on this solver's
         buffers and keep the fastest (``ls_solver_autotune``).  epsilon, if

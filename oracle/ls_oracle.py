@@ -336,7 +336,7 @@ class _RealSpectral:
 def lippmann_schwinger_rfft(lmbda, mu, epsilon_bar, grid_spec, rtol=1e-6, atol=0.0,
                             depth=0, maxiter=1000, workers=None, tau=None,
                             epsilon0=None, rel_error_ref=None, max_iterations_timed=None,
-                            callback=None):
+                            callback=None, stress_law=None):
     """Same iteration as lippmann_schwinger_jax on real FFTs (scipy.fft, float64).
 
     The z-half spectrum and the Hermitian weights of fourier.cu:155-159 replace
@@ -351,6 +351,9 @@ def lippmann_schwinger_rfft(lmbda, mu, epsilon_bar, grid_spec, rtol=1e-6, atol=0
       uses the residual of the start iterate (lippmann_schwinger.py:193-194);
       with a warm start the zero-start residual R(epsilon_bar) is passed here.
     * ``max_iterations_timed``: stop after that many updates (CPU timing).
+    * ``stress_law``: sigma(eps) in place of Hooke's law (the damaged law of
+      elasticity_solve, lippmann_schwinger.py:334-462; the reference medium
+      stays the mid-range of lmbda and mu).
 
     Returns (epsilon, sigma, iter, history) with history the list of R values.
     """
@@ -369,7 +372,7 @@ def lippmann_schwinger_rfft(lmbda, mu, epsilon_bar, grid_spec, rtol=1e-6, atol=0
     u = np.zeros(depth + 1)
 
     def stress(e):
-        s = compute_sigma(lmbda, mu, e.copy())
+        s = compute_sigma(lmbda, mu, e.copy()) if stress_law is None else stress_law(e)
         if tau is not None:
             s += tau
         return s
