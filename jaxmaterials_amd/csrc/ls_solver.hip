@@ -1281,18 +1281,31 @@ int ls_solver_phase_field(ls_solver* s, const double* d_H, const double* d_dold,
     const double A0 = 0.5 * ((inv_lc2 + 2.0 * hmin / gclc) + (inv_lc2 + 2.0 * hmax / gclc));
     k_pf_init<<<nb_mm, kThreads, 0, st>>>(N, d_H, d_dold, inv_lc2, gclc, A0, chi, d_out);
     const FastDiv fy = make_fastdiv(Ty), fx = make_fastdiv(Tx);
-    double res = 1.0;
-    int it = 0;
-    bool ok = hipGetLastError() == hipSuccess;
-    while (ok && res > tol && it < maxiter) {
+    auto one_iteration = [&]() {
       k_pf_z_fwd<<<nblk_z, kPfNT, lds_z, st>>>(s->gd, chi, C, L);
       k_pf_y<false><<<nx * nch_y, kPfNT, lds_y, st>>>(s->gd, C, fy, nch_y);
       k_pf_x<<<(unsigned)nblk_x, kPfNT, lds_x, st>>>(s->gd, tb, C, fx, A0);
       k_pf_y<true><<<nx * nch_y, kPfNT, lds_y, st>>>(s->gd, C, fy, nch_y);
       k_pf_z_inv<<<nblk_z, kPfNT, lds_z, st>>>(s->gd, C, d_H, inv_lc2, gclc, A0, chi, d_out, part, L);
       k_pf_sum<<<1, kThreads, 0, st>>>(part, nblk_z, sums);
+    };
+    // the six launches of an iteration as one captured graph (kernels only),
+    // replayed once per iteration; the stop test reads two doubles each time
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    bool ok = hipGetLastError() == hipSuccess &&
+              hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess;
+    if (ok) {
+      one_iteration();
+      ok = hipStreamEndCapture(st, &graph) == hipSuccess &&
+           hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0) == hipSuccess;
+    }
+    if (graph) (void)hipGraphDestroy(graph);
+    double res = 1.0;
+    int it = 0;
+    while (ok && res > tol && it < maxiter) {
       double h[2];
-      ok = hipGetLastError() == hipSuccess &&
+      ok = hipGraphLaunch(gexec, st) == hipSuccess &&
            hipMemcpyAsync(h, sums, sizeof(h), hipMemcpyDeviceToHost, st) == hipSuccess &&
            hipStreamSynchronize(st) == hipSuccess;
       if (!ok) break;
@@ -1300,6 +1313,7 @@ int ls_solver_phase_field(ls_solver* s, const double* d_H, const double* d_dold,
       res = nc > 1e-12 ? nd / nc : 0.0;
       ++it;
     }
+    if (gexec) (void)hipGraphExecDestroy(gexec);
     if (!ok) break;
     if (iters) *iters = it;
     if (residual) *residual = res;

@@ -85,12 +85,14 @@ def compute_strain_energy(lmbda, mu, epsilon, history=None):
 
 
 def elasticity_solve(lmbda, mu, epsilon_bar, dvar, kk, grid_spec, rtol=1e-6, atol=1e-20, depth=0, maxiter=32,
-                     solver=None):
+                     solver=None, graph=True):
     """Lippmann-Schwinger iteration with Anderson acceleration for the damaged
     law (lippmann_schwinger.py:334-462).  Returns (epsilon, sigma, iter) with
     GPU float32 [6, nx, ny, nz] fields; ``iter`` counts the updates and
     equals maxiter when the tolerance was not reached (no exception, as in the
-    reference).  ``solver``: an LSSolver for this grid and depth to reuse."""
+    reference).  ``solver``: an LSSolver for this grid and depth to reuse;
+    ``graph``: replay the iterations between host synchronisations as
+    captured HIP graphs (fewer launches on small grids)."""
     shape = (grid_spec.nx, grid_spec.ny, grid_spec.nz)
     lam = _f32(lmbda, shape, "lmbda")
     m = _f32(mu, shape, "mu")
@@ -107,7 +109,7 @@ def elasticity_solve(lmbda, mu, epsilon_bar, dvar, kk, grid_spec, rtol=1e-6, ato
     try:
         s.set_damage(d, kk)
         out = s.solve(lam, m, eb.astype(np.float32).reshape(1, 6), want_sigma=True, rtol=rtol,
-                      atol=atol, maxiter=maxiter, check=False)
+                      atol=atol, maxiter=maxiter, check=False, graph=graph)
     finally:
         s.set_damage(None)
         if solver is None:
