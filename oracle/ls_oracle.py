@@ -234,7 +234,10 @@ def relative_divergence_rfft(sigma_hat_half, xi_half, nz):
     num = np.sqrt(np.sum(e * hermitian_weights(nz)))
     s0 = np.real(sigma_hat_half[:, 0, 0, 0])
     den = np.sqrt(np.sum(VOIGT_WEIGHTS * s0**2))
-    return num / den, den
+    # a zero load gives 0 / 0 = nan, as in the reference; its loop then does
+    # not run
+    with np.errstate(invalid="ignore", divide="ignore"):
+        return num / den, den
 
 
 def fourier_solve(tau_hat, lmbda0, mu0, xizero):
