@@ -32,8 +32,11 @@ def model_bytes(kind, N, M, iters, depth, nrebase=0):
     pass and direction; per load case and iteration j (f_j = min(j, depth+1)
     filled Anderson slots when the z pass of iteration j runs):
       z plain    : spectrum 96 M + strain 48 N (+ Lame 8 N shared by the batch)
-      z Anderson : spectrum 96 M + strain 48 N + newest g written 24 N
-                   + older g read (f_j - 1) 24 N (+ Lame 8 N shared)
+      z Anderson : spectrum 96 M + newest g written 24 N + older g read
+                   (f_j - 1) 24 N + once the ring is full (j > depth + 1) the
+                   oldest g read 24 N to rebuild the current correction (+ Lame
+                   8 N shared; the first iteration after a rebase skips that
+                   read, not counted)
       y (inverse + forward) : 2 x 96 M
       x : 120 M per evaluation (setup, every check, every re-evaluation): the
           spectrum read and written and the base divergence d_base read; with
@@ -47,7 +50,8 @@ def model_bytes(kind, N, M, iters, depth, nrebase=0):
     if kind == "z":
         return sum(k * (96 * M + 48 * N) for k in its) + nb * 8 * N if depth == 0 else 0
     if kind == "z_aa":
-        return sum(96 * M + 72 * N + 24 * N * (min(j, m1) - 1) for k in its for j in range(1, k + 1)) + nb * 8 * N
+        return sum(96 * M + 24 * N + 24 * N * (min(j, m1) - 1) + (24 * N if j > m1 else 0)
+                   for k in its for j in range(1, k + 1)) + nb * 8 * N
     if kind == "y":
         return sum(2 * k for k in its) * 96 * M
     if kind == "x":

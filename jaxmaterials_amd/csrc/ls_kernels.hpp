@@ -114,7 +114,7 @@ enum ZMode : int {
   Z_UPD = 2,    // C2R -> delta += r -> sigma = C delta -> R2C
   Z_SET = 3,    // C2R -> eps = eps_bar - r (end of a projection; no forward transform)
   Z_PROJ = 4,   // eps read, sigma = C0 (eps - eps_bar), R2C  (start of a projection)
-  Z_AA = 5,     // C2R -> r; g = delta + r into the newest slot; delta = sum_s alpha_s g^(s) -> C delta -> R2C
+  Z_AA = 5,     // C2R -> r; g = delta + r into the newest slot; next delta = sum_s alpha_s g^(s) -> C delta -> R2C
 };
 
 // Anderson history (lippmann_schwinger.py:177-231, doc/linear_elasticity.tex:335-370).
@@ -129,11 +129,21 @@ enum ZMode : int {
 // (y = sqrt(w_k/2) Q^{1/2} u with Q = |xi~|^2 I + xi~ xi~^T), so the x pass
 // stores y and k_aa_gram streams the ring.  Entries outside the grid's modes
 // stay zero.
+//
+// The correction delta of the current iterate is not stored during the
+// iteration: delta^(k) = sum_s alpha^(k-1)_s g^(s) is rebuilt from the ring,
+// whose operands are all still there when the z pass of iteration k runs
+// (the newest slot keeps the oldest entry until that pass overwrites it).
+// Every such sum runs oldest slot first, newest last, with the same FP32
+// operations in the z pass that forms delta, in the z pass and k_rebase that
+// rebuild it and in k_aa_delta that writes it out, so the rebuilt value is
+// bit-identical to the one whose stress was transformed.
 struct AAParams {
   float* G;             // [R][m1][6][N]
   float2* U;            // [R][m1][u_per_rhs] complex (y per mode, three per mode)
   long long u_per_rhs;  // complex entries per slot (= DbLayout::per_rhs)
   const double* alpha;  // [R][m1] mixing weights by slot
+  const double* alpha_prev;  // [R][m1] weights of the previous mix: delta of the current iterate
   const int* head;      // [R] slot of the newest entry
   const int* filled;    // [R] number of filled slots including the newest
   int m1;               // depth + 1
@@ -301,38 +311,57 @@ __global__ void __launch_bounds__(NT) k_zline(GridDev g, const RhsDesc* __restri
 #pragma unroll
         for (int c = 0; c < 6; ++c) ev[c] = E_[c * N + v];
       } else if constexpr (MODE == Z_AA) {
-        // r from the inverse transform; g = delta + r into the newest slot;
-        // delta = sum_s alpha_s g^(s) (the newest from registers)
+        // delta^(k) = sum_s alpha'_s g^(s) rebuilt from the ring (previous
+        // weights alpha'; the newest slot still holds the oldest entry, a
+        // term of that sum when the ring was full); r from the inverse
+        // transform; g = delta^(k) + r into the newest slot;
+        // delta^(k+1) = sum_s alpha_s g^(s).  Both sums oldest slot first,
+        // each slot read once.
         const int m1 = aa.m1;
         const int f = aa.filled[r];
         const int h = aa.head[r];
         const long long slot_stride = 6 * N;
         float* Gb = aa.G + (long long)r * m1 * slot_stride;
         const double* al = aa.alpha + (long long)r * m1;
+        const double* ap = aa.alpha_prev + (long long)r * m1;
+        float* Gh = Gb + h * slot_stride;
+        float dk[6], dn[6];
+        const double aph = ap[h];
+        if (aph != 0.0) {
+          const float a = (float)aph;
+#pragma unroll
+          for (int c = 0; c < 6; ++c) dk[c] = a * Gh[c * N + v];
+        } else {
+#pragma unroll
+          for (int c = 0; c < 6; ++c) dk[c] = 0.f;
+        }
+#pragma unroll
+        for (int c = 0; c < 6; ++c) dn[c] = 0.f;
+        for (int s2 = f - 1; s2 >= 1; --s2) {
+          int sl = h - s2;
+          if (sl < 0) sl += m1;
+          const float a = (float)al[sl], b = (float)ap[sl];
+          const float* Gs = Gb + sl * slot_stride;
+#pragma unroll
+          for (int c = 0; c < 6; ++c) {
+            const float gv = Gs[c * N + v];
+            dn[c] = fmaf(a, gv, dn[c]);
+            dk[c] = fmaf(b, gv, dk[c]);
+          }
+        }
         float gn[6];
 #pragma unroll
         for (int p = 0; p < 3; ++p) {
           const float2 z = lds[lds_idx<true>(l * 3 + p, j, 1, ls)];
-          gn[2 * p] = E_[(2 * p) * N + v] + z.x;
-          gn[2 * p + 1] = E_[(2 * p + 1) * N + v] + z.y;
+          gn[2 * p] = dk[2 * p] + z.x;
+          gn[2 * p + 1] = dk[2 * p + 1] + z.y;
         }
-        float* Gh = Gb + h * slot_stride;
         const float a0 = (float)al[h];
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
           Gh[c * N + v] = gn[c];
-          ev[c] = a0 * gn[c];
+          ev[c] = fmaf(a0, gn[c], dn[c]);
         }
-        for (int s2 = 1; s2 < f; ++s2) {
-          int sl = h - s2;
-          if (sl < 0) sl += m1;
-          const float a = (float)al[sl];
-          const float* Gs = Gb + sl * slot_stride;
-#pragma unroll
-          for (int c = 0; c < 6; ++c) ev[c] = fmaf(a, Gs[c * N + v], ev[c]);
-        }
-#pragma unroll
-        for (int c = 0; c < 6; ++c) E_[c * N + v] = ev[c];
       } else {
         float rr[6];
 #pragma unroll
@@ -725,25 +754,96 @@ __global__ void __launch_bounds__(kThreads) k_check(RhsState* __restrict__ st, c
 }
 
 // ---------------------------------------------------------------------------
-// Move the correction into the base: base += delta, delta = 0; the Anderson
-// history g^(s) = delta^(s) + r^(s) is re-expressed relative to the new base.
+// delta of the current iterate from the Anderson ring, sum_s c_s g^(s) with
+// c = alpha' (weights of the previous mix) or, with newest_last, c = alpha
+// (the mix the last z pass applied: its newest operand sits in slot h).
+// aa_ring_load reads each filled slot once (gv[a] = slot h - a, all loads
+// issued before the sum); aa_ring_sum adds the terms with c_s != 0 in the
+// order of the z pass (oldest first).
+__device__ __forceinline__ int aa_slot(int h, int a, int m1) { return h - a < 0 ? h - a + m1 : h - a; }
+
+__device__ __forceinline__ void aa_ring_load(const float* Gb, long long ss, long long i, int h, int f, int m1,
+                                             float (&gv)[kMaxDepth1]) {
+#pragma unroll
+  for (int a = 0; a < kMaxDepth1; ++a) gv[a] = a < f ? Gb[(long long)aa_slot(h, a, m1) * ss + i] : 0.f;
+}
+
+__device__ __forceinline__ float aa_ring_sum(const float (&gv)[kMaxDepth1], const double* c, int h, int f, int m1,
+                                             bool newest_last) {
+  float acc = 0.f;
+  if (!newest_last && f > 0 && c[h] != 0.0) acc = (float)c[h] * gv[0];
+#pragma unroll
+  for (int a = kMaxDepth1 - 1; a >= 1; --a) {
+    if (a < f) {
+      const double ca = c[aa_slot(h, a, m1)];
+      if (ca != 0.0) acc = fmaf((float)ca, gv[a], acc);
+    }
+  }
+  if (newest_last && f > 0 && c[h] != 0.0) acc = fmaf((float)c[h], gv[0], acc);
+  return acc;
+}
+
+// Move the correction into the base: base += delta, delta = 0.  Plain
+// iteration: delta from its buffer.  Anderson (G given): delta rebuilt from
+// the ring with the previous weights, and the history g^(s) = delta^(s) + r^(s)
+// re-expressed relative to the new base (the caller then clears alpha', so the
+// next z pass rebuilds delta = 0).
 __global__ void __launch_bounds__(kThreads) k_rebase(const RhsState* __restrict__ st, long long N,
                                                      float* __restrict__ base, float* __restrict__ delta,
-                                                     float* __restrict__ G, const int* __restrict__ filled, int m1) {
+                                                     float* __restrict__ G, const int* __restrict__ head,
+                                                     const int* __restrict__ filled,
+                                                     const double* __restrict__ alpha_prev, int m1) {
   const int r = blockIdx.y;
   if (!st[r].active) return;
   const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
   if (i >= 6 * N) return;
   const long long off = (long long)r * 6 * N + i;
-  const float dl = delta[off];
-  base[off] += dl;
-  delta[off] = 0.f;
-  if (G) {
-    const int f = filled[r];
-    float* Gb = G + (long long)r * m1 * 6 * N;
-    for (int s = 0; s < m1; ++s)
-      if (s < f) Gb[(long long)s * 6 * N + i] -= dl;
+  if (!G) {
+    base[off] += delta[off];
+    delta[off] = 0.f;
+    return;
   }
+  const int f = filled[r], h = head[r];
+  float* Gb = G + (long long)r * m1 * 6 * N;
+  const long long ss = 6 * N;
+  float gv[kMaxDepth1];
+  aa_ring_load(Gb, ss, i, h, f, m1, gv);
+  const float dl = aa_ring_sum(gv, alpha_prev + (long long)r * m1, h, f, m1, false);
+  base[off] += dl;
+#pragma unroll
+  for (int a = 0; a < kMaxDepth1; ++a)
+    if (a < f) Gb[(long long)aa_slot(h, a, m1) * ss + i] = gv[a] - dl;
+}
+
+// delta of every selected load case written from the Anderson ring (end of
+// a solve, or before the stress of the reference's last-update quirk): with
+// the weights of the last z pass for the load cases still active (their last
+// step was a z pass), else with the previous weights (their last step was a
+// residual evaluation after the ring advanced).
+__global__ void __launch_bounds__(kThreads) k_aa_delta(const RhsState* __restrict__ st,
+                                                       const int* __restrict__ sel, long long N,
+                                                       const float* __restrict__ G, const int* __restrict__ head,
+                                                       const int* __restrict__ filled,
+                                                       const double* __restrict__ alpha,
+                                                       const double* __restrict__ alpha_prev, int m1,
+                                                       int last_if_active, float* __restrict__ delta) {
+  const int r = blockIdx.y;
+  if (sel && !sel[r]) return;
+  const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= 6 * N) return;
+  const bool cur = last_if_active && st[r].active;
+  const double* c = (cur ? alpha : alpha_prev) + (long long)r * m1;
+  const int h = head[r], f = filled[r];
+  float gv[kMaxDepth1];
+  aa_ring_load(G + (long long)r * m1 * 6 * N, 6 * N, i, h, f, m1, gv);
+  delta[(long long)r * 6 * N + i] = aa_ring_sum(gv, c, h, f, m1, cur);
+}
+
+// alpha' = 0 for the active load cases (after a rebase: delta = 0)
+__global__ void k_aa_forget(const RhsState* __restrict__ st, double* __restrict__ alpha_prev, int nrhs, int m1) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrhs || !st[r].active) return;
+  for (int s = 0; s < m1; ++s) alpha_prev[(long long)r * m1 + s] = 0.0;
 }
 
 // For the load cases with sel[r] != 0: base = src (or eps_bar if src is
@@ -1018,12 +1118,16 @@ __global__ void __launch_bounds__(kThreads) k_aa_solve(const RhsState* __restric
 }
 
 // Advance the Anderson ring of every active load case before the x pass that
-// records a new residual: newest slot, fill count, and the Gram matrix shifted
-// by one age (row and column 0 are then set by k_aa_solve).
+// records a new residual: previous weights, newest slot, fill count, and the
+// Gram matrix shifted by one age (row and column 0 are then set by
+// k_aa_solve).
 __global__ void k_aa_advance(const RhsState* __restrict__ st, int* __restrict__ head, int* __restrict__ filled,
-                             double* __restrict__ gram, int nrhs, int m1) {
+                             double* __restrict__ gram, const double* __restrict__ alpha,
+                             double* __restrict__ alpha_prev, int nrhs, int m1) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nrhs || !st[r].active) return;
+  // the weights of the last mix rebuild the current iterate's delta
+  for (int s = 0; s < m1; ++s) alpha_prev[(long long)r * m1 + s] = alpha[(long long)r * m1 + s];
   head[r] = (head[r] + 1) % m1;
   filled[r] = min(filled[r] + 1, m1);
   double* A = gram + (long long)r * m1 * m1;

@@ -231,6 +231,7 @@ struct ls_solver {
   int nblk_p = 0;           // blocks per load case of the plane correction
   long long chunk_p = 0;
   double* aa_alpha = nullptr;
+  double* aa_alpha_prev = nullptr;  // weights of the previous mix (rebuild delta)
   double* aa_dots = nullptr;
   double* aa_gram = nullptr;
   int* aa_head = nullptr;
@@ -350,6 +351,7 @@ AAParams aa_params(const ls_solver* s, int record) {
   aa.U = s->aaU;
   aa.u_per_rhs = s->dl.per_rhs;
   aa.alpha = s->aa_alpha;
+  aa.alpha_prev = s->aa_alpha_prev;
   aa.head = s->aa_head;
   aa.filled = s->aa_filled;
   aa.m1 = s->m1;
@@ -541,7 +543,7 @@ void free_solver(ls_solver* s) {
   }
   void* dev[] = {s->spec,    s->spec64,  s->base,    s->dbase,    s->partials, s->dc,       s->d_rhs,
                  s->d_st,    s->d_sel,   s->part6,   s->d_mean6,  s->d_mm,     s->aaG,      s->aaU,      s->aaPl,
-                 s->aa_alpha, s->aa_dots, s->aa_gram, s->aa_head, s->aa_filled};
+                 s->aa_alpha, s->aa_alpha_prev, s->aa_dots, s->aa_gram, s->aa_head, s->aa_filled};
   for (void* p : dev) (void)hipFree(p);
   (void)hipHostFree(s->h_st);
   (void)hipHostFree(s->h_mean6);
@@ -696,6 +698,7 @@ int create_impl(ls_solver* s, const int* voxels, const float* extents, int nrhs,
     s->chunk_p = (s->pl_slot + s->nblk_p - 1) / s->nblk_p;
     if (dalloc(&s->aaG, hist) || dalloc(&s->aaU, (size_t)nrhs * s->m1 * s->dl.per_rhs) ||
         dalloc(&s->aaPl, (size_t)nrhs * s->m1 * s->pl_slot) || dalloc(&s->aa_alpha, (size_t)nrhs * s->m1) ||
+        dalloc(&s->aa_alpha_prev, (size_t)nrhs * s->m1) ||
         dalloc(&s->aa_dots, (size_t)nrhs * (s->nblk_g + s->nblk_p) * s->m1) ||
         dalloc(&s->aa_gram, (size_t)nrhs * s->m1 * s->m1) ||
         dalloc(&s->aa_head, nrhs) || dalloc(&s->aa_filled, nrhs))
@@ -716,6 +719,9 @@ int reset_aa(ls_solver* s) {
   // entries of the y ring outside the grid's modes are never written: keep them zero
   LS_HIP(hipMemsetAsync(s->aaU, 0, (size_t)s->nrhs * s->m1 * s->dl.per_rhs * sizeof(float2), s->stream));
   LS_HIP(hipMemsetAsync(s->aaPl, 0, (size_t)s->nrhs * s->m1 * s->pl_slot * sizeof(float), s->stream));
+  // no weights yet: the first rebuilt delta is 0
+  LS_HIP(hipMemsetAsync(s->aa_alpha, 0, (size_t)s->nrhs * s->m1 * sizeof(double), s->stream));
+  LS_HIP(hipMemsetAsync(s->aa_alpha_prev, 0, (size_t)s->nrhs * s->m1 * sizeof(double), s->stream));
   LS_HIP(hipStreamSynchronize(s->stream));
   return 0;
 }
@@ -732,7 +738,8 @@ void init_fields(ls_solver* s, const int* d_sel, float* delta, const float* src)
 // weights follow the inverse y pass of the next iteration (aa_weights).
 void eval_x(ls_solver* s, double rtol, double atol, int mode, bool advance) {
   if (s->depth > 0 && advance)
-    k_aa_advance<<<(s->nrhs + 63) / 64, 64, 0, s->stream>>>(s->d_st, s->aa_head, s->aa_filled, s->aa_gram, s->nrhs,
+    k_aa_advance<<<(s->nrhs + 63) / 64, 64, 0, s->stream>>>(s->d_st, s->aa_head, s->aa_filled, s->aa_gram,
+                                                           s->aa_alpha, s->aa_alpha_prev, s->nrhs,
                                                            s->m1);
   xpass(s, 1, true, s->depth > 0);
   check(s, rtol, atol, mode);
@@ -805,12 +812,24 @@ int iterate_graph(ls_solver* s, const float* lam, const float* mu, float* eps, d
   return 0;
 }
 
+// The Anderson iteration keeps delta in its ring (k_aa_delta): write it into
+// its buffer for the selected load cases.
+void aa_write_delta(ls_solver* s, const int* d_sel, float* eps, int last_if_active) {
+  if (s->depth == 0) return;
+  dim3 grid((unsigned)((6 * s->N + kThreads - 1) / kThreads), s->nrhs);
+  k_aa_delta<<<grid, kThreads, 0, s->stream>>>(s->d_st, d_sel, s->N, s->aaG, s->aa_head, s->aa_filled, s->aa_alpha,
+                                               s->aa_alpha_prev, s->m1, last_if_active, eps);
+}
+
 // Move the corrections of all active load cases into their bases and
 // re-evaluate (host state copy must be current).
 int rebase(ls_solver* s, const float* lam, const float* mu, const float* tau, float* eps, double rtol,
            double atol) {
   dim3 grid((unsigned)((6 * s->N + kThreads - 1) / kThreads), s->nrhs);
-  k_rebase<<<grid, kThreads, 0, s->stream>>>(s->d_st, s->N, s->base, eps, s->aaG, s->aa_filled, s->m1);
+  k_rebase<<<grid, kThreads, 0, s->stream>>>(s->d_st, s->N, s->base, eps, s->aaG, s->aa_head, s->aa_filled,
+                                              s->aa_alpha_prev, s->m1);
+  if (s->depth > 0)
+    k_aa_forget<<<(s->nrhs + 63) / 64, 64, 0, s->stream>>>(s->d_st, s->aa_alpha_prev, s->nrhs, s->m1);
   base_eval_active(s, lam, mu, tau);
   if (zero_spec(s, active_mask(s))) return -1;
   eval_x(s, rtol, atol, 2, false);
@@ -995,6 +1014,7 @@ int solve_impl(ls_solver* s, const float* d_lambda, const float* d_mu, const int
       // computed before the final update (lippmann_schwinger.cu:192-216, 230-231)
       for (int r = 0; r < s->nrhs; ++r) q7[r] = s->h_st[r].active ? 1 : 0;
       if (put_sel(s, q7)) return -1;
+      aa_write_delta(s, s->d_sel, d_eps, 0);
       dim3 grid(s->nblk_stress, s->nrhs);
       k_stress<<<grid, kThreads, 0, s->stream>>>(s->gd, s->d_rhs, s->d_sel, d_lambda, d_mu, s->base, d_eps, 0,
                                                   d_tau, d_sigma, s->part6, s->chunk_stress);
@@ -1019,6 +1039,7 @@ int solve_impl(ls_solver* s, const float* d_lambda, const float* d_mu, const int
     if (q7_done)
       for (int r = 0; r < s->nrhs; ++r) sel[r] = q7[r] ? 0 : 1;
     if (put_sel(s, sel)) return -1;
+    aa_write_delta(s, nullptr, d_eps, 1);
     k_stress<<<grid, kThreads, 0, s->stream>>>(s->gd, s->d_rhs, s->d_sel, d_lambda, d_mu, s->base, d_eps, 1, d_tau,
                                                 d_sigma, s->part6, s->chunk_stress);
     if (q7_done) {
