@@ -220,7 +220,7 @@ struct ls_solver {
   float* h_mm = nullptr;  // pinned
   int nblk_stress = 0, chunk_stress = 0;
   int nblk_mm = 0, chunk_mm = 0;
-  double rebase_factor = 1e-3;
+  double rebase_factor = 1e-4;  // LS_REBASE_FACTOR; one rebase per 256^3 solve to 1e-6
   int n_rebase = 0;
   int absolute = 0;  // LS_ABSOLUTE_RESIDUAL flag of the current solve
   // Anderson
