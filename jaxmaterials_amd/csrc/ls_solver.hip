@@ -606,8 +606,10 @@ int create_impl(ls_solver* s, const int* voxels, const float* extents, int nrhs,
   const int n[3] = {s->nx, s->ny, s->nz};
   for (int d = 0; d < 3; ++d)
     if (build_axis(s->ax[d], n[d])) return -1;
-  // tiles: aim for ~3-6k complex elements per block (overridable for tuning)
-  const int wz = env_int("LS_TILE_Z", std::max(1, 3072 / (3 * s->nz)));
+  // tiles: aim for ~3-6k complex elements per block in y and x, ~1.5k in z
+  // (two lines of six components at 256: 2% faster z passes than four)
+  // (overridable for tuning)
+  const int wz = env_int("LS_TILE_Z", std::max(1, 1536 / (3 * s->nz)));
   const int wy = env_int("LS_TILE_Y", std::max(1, std::min(6 * s->P, 4096 / s->ny)));
   const int wx = env_int("LS_TILE_X", std::max(1, 3072 / (6 * s->nx)));
   const int lsz = s->nz + (s->nz >> 4) + 1;  // padded z-line LDS stride
