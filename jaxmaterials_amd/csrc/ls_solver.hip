@@ -112,6 +112,7 @@ AxisPlan to_dev(const HostAxis& a) {
 
 constexpr size_t kMaxLds = 60 * 1024;
 constexpr int kMaxTm = 256;  // largest x-pass tile (modes per block); bounds the d_base padding
+constexpr int kCUs = 256;     // compute units of an MI355X: the default tiles leave one block per CU on small grids
 
 struct Tile {
   int lines = 0;  // Lz (z), T (y) or Tm (x)
@@ -607,11 +608,16 @@ int create_impl(ls_solver* s, const int* voxels, const float* extents, int nrhs,
   for (int d = 0; d < 3; ++d)
     if (build_axis(s->ax[d], n[d])) return -1;
   // tiles: aim for ~3-6k complex elements per block in y and x, ~1.5k in z
-  // (two lines of six components at 256: 2% faster z passes than four)
+  // (two lines of six components at 256: 2% faster z passes than four), but
+  // on small grids no more lines than leave about one block per CU: there a
+  // pass is latency-bound and more, smaller blocks finish sooner
   // (overridable for tuning)
-  const int wz = env_int("LS_TILE_Z", std::max(1, 1536 / (3 * s->nz)));
-  const int wy = env_int("LS_TILE_Y", std::max(1, std::min(6 * s->P, 4096 / s->ny)));
-  const int wx = env_int("LS_TILE_X", std::max(1, 3072 / (6 * s->nx)));
+  auto per_cu = [](int w, long long units) {
+    return std::max(1, (int)std::min<long long>(w, std::max(1LL, units / kCUs)));
+  };
+  const int wz = env_int("LS_TILE_Z", per_cu(std::max(1, 1536 / (3 * s->nz)), (long long)s->nx * s->ny));
+  const int wy = env_int("LS_TILE_Y", per_cu(std::max(1, std::min(6 * s->P, 4096 / s->ny)), 6LL * s->nx * s->P));
+  const int wx = env_int("LS_TILE_X", per_cu(std::max(1, 3072 / (6 * s->nx)), (long long)s->ny * s->P));
   const int lsz = s->nz + (s->nz >> 4) + 1;  // padded z-line LDS stride
   const bool ok32 = choose_tile(s->ax[2], wz, 3, (size_t)3 * lsz * sizeof(float2), kBudget32, s->tz) &&
                     choose_tile(s->ax[1], wy, 1, (size_t)s->ny * sizeof(float2), kBudget32, s->ty) &&
@@ -1242,15 +1248,16 @@ int ls_solver_phase_field(ls_solver* s, const double* d_H, const double* d_dold,
   }
   const long long N = s->N;
   hipStream_t st = s->stream;
-  // tiles of the five passes (lines x length <= kPfItems per block)
-  const int L = std::max(1, std::min(1024, kPfItems / nz));
+  // tiles of the five passes (lines x length <= kPfItems per block; on small
+  // grids no more lines than leave about one block per CU)
+  const int L = std::max(1, std::min({1024, kPfItems / nz, nx * ny / kCUs}));
   const int nblk_z = (nx * ny + L - 1) / L;
   const size_t lds_z = (size_t)L * s->gd.lsz * sizeof(double2);
-  const int Ty = std::max(1, std::min(nz, kPfItems / ny));
+  const int Ty = std::max(1, std::min({nz, kPfItems / ny, nx * nz / kCUs}));
   const int nch_y = (nz + Ty - 1) / Ty;
   const size_t lds_y = (size_t)ny * Ty * sizeof(double2);
   const long long Q = (long long)ny * nz;
-  const int Tx = (int)std::max(1LL, std::min(Q, (long long)(kPfItems / nx)));
+  const int Tx = (int)std::max(1LL, std::min({Q, (long long)(kPfItems / nx), Q / kCUs}));
   const long long nblk_x = (Q + Tx - 1) / Tx;
   const size_t lds_x = (size_t)nx * Tx * sizeof(double2);
   const int nb_mm = (int)std::min<long long>(1024, (N + kThreads - 1) / kThreads);
