@@ -219,6 +219,33 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
   return s;  // valid in thread 0
 }
 
+// Block sums of the first n <= kMaxDepth1 values of v in one pass (the same
+// fixed order as block_sum, one pair of barriers for all of them); valid in
+// thread 0.  red: kThreads / 64 * kMaxDepth1 doubles.
+__device__ __forceinline__ void block_sum_n(double (&v)[kMaxDepth1], int n, double* red) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < kMaxDepth1; ++q)
+    if (q < n) v[q] = wave_sum(v[q]);
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < kMaxDepth1; ++q)
+      if (q < n) red[w * kMaxDepth1 + q] = v[q];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < kMaxDepth1; ++q) {
+      if (q < n) {
+        double s = 0.0;
+        for (int i = 0; i < kThreads / 64; ++i) s += red[i * kMaxDepth1 + q];
+        v[q] = s;
+      }
+    }
+  }
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
 // Remap the linear workgroup id so that every XCD works through a contiguous
 // range of tiles and neighbouring tiles (adjacent HBM segments) share an L2.
@@ -978,7 +1005,7 @@ __global__ void __launch_bounds__(kThreads) k_minmax(const float* __restrict__ l
 // (nblk, R), `chunk` complex entries per block.
 __global__ void __launch_bounds__(kThreads) k_aa_gram(const RhsState* __restrict__ st, AAParams aa,
                                                       double* __restrict__ dots, long long chunk, int stride) {
-  __shared__ double red[kThreads / 64];
+  __shared__ double red[kThreads / 64 * kMaxDepth1];
   const int r = blockIdx.y;
   if (!st[r].active) return;
   const int hd = aa.head[r], f = aa.filled[r];
@@ -1001,10 +1028,9 @@ __global__ void __launch_bounds__(kThreads) k_aa_gram(const RhsState* __restrict
       }
     }
   }
-  for (int q = 0; q < aa.m1; ++q) {
-    const double b = block_sum(acc[q], red);
-    if (threadIdx.x == 0) dots[((long long)r * stride + blockIdx.x) * aa.m1 + q] = (q < f) ? b : 0.0;
-  }
+  block_sum_n(acc, aa.m1, red);
+  if (threadIdx.x == 0)
+    for (int q = 0; q < aa.m1; ++q) dots[((long long)r * stride + blockIdx.x) * aa.m1 + q] = (q < f) ? acc[q] : 0.0;
 }
 
 // Correction of the Gram row for the self-conjugate planes (see AAPlanes):
@@ -1013,7 +1039,7 @@ __global__ void __launch_bounds__(kThreads) k_aa_gram(const RhsState* __restrict
 __global__ void __launch_bounds__(kThreads) k_aa_gram_planes(const RhsState* __restrict__ st, AAPlanes pl,
                                                              const int* __restrict__ filled, double* __restrict__ dots,
                                                              long long chunk, int stride, int boff, double scale) {
-  __shared__ double red[kThreads / 64];
+  __shared__ double red[kThreads / 64 * kMaxDepth1];
   const int r = blockIdx.y;
   if (!st[r].active) return;
   const int hd = pl.head[r], f = filled[r];
@@ -1035,10 +1061,10 @@ __global__ void __launch_bounds__(kThreads) k_aa_gram_planes(const RhsState* __r
       }
     }
   }
-  for (int q = 0; q < pl.m1; ++q) {
-    const double b = block_sum(acc[q], red);
-    if (threadIdx.x == 0) dots[((long long)r * stride + boff + blockIdx.x) * pl.m1 + q] = (q < f) ? -scale * b : 0.0;
-  }
+  block_sum_n(acc, pl.m1, red);
+  if (threadIdx.x == 0)
+    for (int q = 0; q < pl.m1; ++q)
+      dots[((long long)r * stride + boff + blockIdx.x) * pl.m1 + q] = (q < f) ? -scale * acc[q] : 0.0;
 }
 
 // Anderson weights per load case (lippmann_schwinger.py:217-228): reduce the
@@ -1053,22 +1079,29 @@ __global__ void __launch_bounds__(kThreads) k_aa_solve(const RhsState* __restric
                                                        double* __restrict__ gram, double* __restrict__ alpha,
                                                        const int* __restrict__ head,
                                                        const int* __restrict__ filled) {
-  __shared__ double red[kThreads / 64];
-  __shared__ double row[kMaxDepth1];
+  __shared__ double red[kThreads / 64 * kMaxDepth1];
   const int r = blockIdx.x;
   if (!st[r].active) return;
   const int f = filled[r];
-  for (int s = 0; s < m1; ++s) {
-    double a = 0.0;
-    for (int i = threadIdx.x; i < nblk; i += kThreads) a += dots[((long long)r * nblk + i) * m1 + s];
-    const double bs = block_sum(a, red);
-    if (threadIdx.x == 0) row[s] = (s < f) ? bs : 0.0;
+  double row[kMaxDepth1];
+#pragma unroll
+  for (int s = 0; s < kMaxDepth1; ++s) row[s] = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += kThreads) {
+    const double* di = dots + ((long long)r * nblk + i) * m1;
+#pragma unroll
+    for (int s = 0; s < kMaxDepth1; ++s)
+      if (s < m1) row[s] += di[s];
   }
+  block_sum_n(row, m1, red);
   if (threadIdx.x != 0) return;
   double* A = gram + (long long)r * m1 * m1;
-  for (int s = 0; s < m1; ++s) {
-    A[s] = row[s];
-    A[s * m1] = row[s];
+#pragma unroll
+  for (int s = 0; s < kMaxDepth1; ++s) {
+    if (s < m1) {
+      const double v = s < f ? row[s] : 0.0;
+      A[s] = v;
+      A[s * m1] = v;
+    }
   }
   for (int s = f; s < m1; ++s) A[s * m1 + s] = 1.0;  // unfilled ages keep the identity
   double M[kMaxDepth1][kMaxDepth1 + 1];
