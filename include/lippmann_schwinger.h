@@ -173,11 +173,13 @@ int ls_solver_set_damage(ls_solver *h, const float *d_damage, float k);
  * A0 = (min A + max A) / 2, from chi = B - (A - A0) d_old until
  * ||chi_new - chi|| / ||chi_new|| <= tol or maxiter iterations.  Device
  * FP64 fields d_H (history), d_dold, d_out [N]; the iteration count and the
- * last residual go to *iters and *residual.  Every axis <= 3840.  0 on
- * success. */
+ * last residual go to *iters and *residual.  extents: host double[3], the
+ * domain size in FP64 as the reference builds xi~ for this solve (NULL: the
+ * handle's float extents).  Every axis <= 3840.  0 on success. */
 int ls_solver_phase_field(ls_solver *h, const double *d_H, const double *d_dold,
                           double gc, double lc, double tol, int maxiter,
-                          double *d_out, int *iters, double *residual);
+                          double *d_out, int *iters, double *residual,
+                          const double *extents);
 
 /* compute_sigma_damaged (lippmann_schwinger.py:36-74) pointwise on device
  * arrays: d_eps, d_sigma [6][N] float32, d_lambda, d_mu [N], d_damage [N]

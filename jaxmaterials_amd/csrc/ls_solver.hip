@@ -1239,8 +1239,12 @@ ls_solver* ls_solver_create(const int* voxels, const float* extents, int nrhs, i
 void ls_solver_destroy(ls_solver* s) { free_solver(s); }
 
 int ls_solver_phase_field(ls_solver* s, const double* d_H, const double* d_dold, double gc, double lc, double tol,
-                          int maxiter, double* d_out, int* iters, double* residual) {
-  if (!s || !d_H || !d_dold || !d_out || !(gc > 0.0) || !(lc > 0.0) || maxiter < 0) {
+                          int maxiter, double* d_out, int* iters, double* residual, const double* extents) {
+  const double ext[3] = {!s ? 0.0 : extents ? extents[0] : (double)s->Lx,
+                         !s ? 0.0 : extents ? extents[1] : (double)s->Ly,
+                         !s ? 0.0 : extents ? extents[2] : (double)s->Lz};
+  if (!s || !d_H || !d_dold || !d_out || !(gc > 0.0) || !(lc > 0.0) || maxiter < 0 ||
+      !(ext[0] > 0.0 && ext[1] > 0.0 && ext[2] > 0.0)) {
     fprintf(stderr, "liblippmannschwinger: ls_solver_phase_field: invalid arguments\n");
     return -1;
   }
@@ -1285,7 +1289,7 @@ int ls_solver_phase_field(ls_solver* s, const double* d_H, const double* d_dold,
   int rc = -1;
   do {
     if (dalloc(&C, (size_t)N) || dalloc(&chi, (size_t)N) ||
-        dalloc(&part, (size_t)2 * std::max<long long>(nblk_z, nb_mm)) || dalloc(&sums, 2) ||
+        dalloc(&part, (size_t)2 * std::max<long long>(nblk_z, nb_mm)) || dalloc(&sums, 4) ||
         dalloc(&dtab, tab.size()))
       break;
     if (hipMemcpyAsync(dtab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess) break;
@@ -1296,9 +1300,9 @@ int ls_solver_phase_field(ls_solver* s, const double* d_H, const double* d_dold,
     tb.cy = dtab + off[1] + ny;
     tb.sz = dtab + off[2];
     tb.cz = dtab + off[2] + nz;
-    tb.gx = 2.0 * nx / (double)s->Lx;
-    tb.gy = 2.0 * ny / (double)s->Ly;
-    tb.gz = 2.0 * nz / (double)s->Lz;
+    tb.gx = 2.0 * nx / ext[0];
+    tb.gy = 2.0 * ny / ext[1];
+    tb.gz = 2.0 * nz / ext[2];
     // A0 from min / max of H (A increases with H)
     k_pf_minmax<<<nb_mm, kThreads, 0, st>>>(d_H, N, part);
     std::vector<double> mm(2 * (size_t)nb_mm);
@@ -1314,39 +1318,51 @@ int ls_solver_phase_field(ls_solver* s, const double* d_H, const double* d_dold,
     const double A0 = 0.5 * ((inv_lc2 + 2.0 * hmin / gclc) + (inv_lc2 + 2.0 * hmax / gclc));
     k_pf_init<<<nb_mm, kThreads, 0, st>>>(N, d_H, d_dold, inv_lc2, gclc, A0, chi, d_out);
     const FastDiv fy = make_fastdiv(Ty), fx = make_fastdiv(Tx);
+    // state[0] stop flag, state[1] iterations (k_pf_check); sums[2] the residual
+    int* state = reinterpret_cast<int*>(sums + 3);
     auto one_iteration = [&]() {
-      k_pf_z_fwd<<<nblk_z, kPfNT, lds_z, st>>>(s->gd, chi, C, L);
-      k_pf_y<false><<<nx * nch_y, kPfNT, lds_y, st>>>(s->gd, C, fy, nch_y);
-      k_pf_x<<<(unsigned)nblk_x, kPfNT, lds_x, st>>>(s->gd, tb, C, fx, A0);
-      k_pf_y<true><<<nx * nch_y, kPfNT, lds_y, st>>>(s->gd, C, fy, nch_y);
-      k_pf_z_inv<<<nblk_z, kPfNT, lds_z, st>>>(s->gd, C, d_H, inv_lc2, gclc, A0, chi, d_out, part, L);
-      k_pf_sum<<<1, kThreads, 0, st>>>(part, nblk_z, sums);
+      k_pf_z_fwd<<<nblk_z, kPfNT, lds_z, st>>>(s->gd, state, chi, C, L);
+      k_pf_y<false><<<nx * nch_y, kPfNT, lds_y, st>>>(s->gd, state, C, fy, nch_y);
+      k_pf_x<<<(unsigned)nblk_x, kPfNT, lds_x, st>>>(s->gd, state, tb, C, fx, A0);
+      k_pf_y<true><<<nx * nch_y, kPfNT, lds_y, st>>>(s->gd, state, C, fy, nch_y);
+      k_pf_z_inv<<<nblk_z, kPfNT, lds_z, st>>>(s->gd, state, C, d_H, inv_lc2, gclc, A0, chi, d_out, part, L);
+      k_pf_sum<<<1, kThreads, 0, st>>>(state, part, nblk_z, sums);
+      k_pf_check<<<1, 64, 0, st>>>(sums, tol, maxiter, state, sums + 2);
     };
-    // the six launches of an iteration as one captured graph (kernels only),
-    // replayed once per iteration; the stop test reads two doubles each time
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    bool ok = hipGetLastError() == hipSuccess &&
-              hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess;
-    if (ok) {
-      one_iteration();
-      ok = hipStreamEndCapture(st, &graph) == hipSuccess &&
-           hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0) == hipSuccess;
-    }
-    if (graph) (void)hipGraphDestroy(graph);
+    // a chunk of iterations, each followed by its stop test on the device, as
+    // one captured graph (kernels only); the host reads the state once per
+    // replay.  The loop runs while res > tol and it < maxiter, res = 1 at the
+    // start (lippmann_schwinger.py:540-545).
+    const int chunk = std::max(1, env_int("LS_PF_CHUNK", 8));
     double res = 1.0;
     int it = 0;
-    while (ok && res > tol && it < maxiter) {
-      double h[2];
-      ok = hipGraphLaunch(gexec, st) == hipSuccess &&
-           hipMemcpyAsync(h, sums, sizeof(h), hipMemcpyDeviceToHost, st) == hipSuccess &&
-           hipStreamSynchronize(st) == hipSuccess;
-      if (!ok) break;
-      const double nd = std::sqrt(h[0]), nc = std::sqrt(h[1]);
-      res = nc > 1e-12 ? nd / nc : 0.0;
-      ++it;
+    bool ok = hipGetLastError() == hipSuccess;
+    if (ok && res > tol && maxiter > 0) {
+      ok = hipMemsetAsync(state, 0, 2 * sizeof(int), st) == hipSuccess;
+      hipGraph_t graph = nullptr;
+      hipGraphExec_t gexec = nullptr;
+      ok = ok && hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess;
+      if (ok) {
+        for (int k = 0; k < chunk; ++k) one_iteration();
+        ok = hipStreamEndCapture(st, &graph) == hipSuccess &&
+             hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0) == hipSuccess;
+      }
+      if (graph) (void)hipGraphDestroy(graph);
+      while (ok) {
+        // residual (sums[2]) and the state (sums[3]) in one copy
+        double hb[2];
+        ok = hipGraphLaunch(gexec, st) == hipSuccess &&
+             hipMemcpyAsync(hb, sums + 2, sizeof(hb), hipMemcpyDeviceToHost, st) == hipSuccess &&
+             hipStreamSynchronize(st) == hipSuccess;
+        if (!ok) break;
+        int h[2];
+        std::memcpy(h, &hb[1], sizeof(h));
+        res = hb[0];
+        it = h[1];
+        if (h[0]) break;
+      }
+      if (gexec) (void)hipGraphExecDestroy(gexec);
     }
-    if (gexec) (void)hipGraphExecDestroy(gexec);
     if (!ok) break;
     if (iters) *iters = it;
     if (residual) *residual = res;

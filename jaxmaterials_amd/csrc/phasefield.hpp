@@ -54,8 +54,9 @@ __global__ void __launch_bounds__(kThreads) k_pf_init(long long N, const double*
 }
 
 // z lines, forward, from the real chi: L lines of nz per block
-__global__ void __launch_bounds__(kPfNT) k_pf_z_fwd(GridDev g, const double* __restrict__ chi,
-                                                    double2* __restrict__ C, int L) {
+__global__ void __launch_bounds__(kPfNT) k_pf_z_fwd(GridDev g, const int* __restrict__ stop,
+                                                    const double* __restrict__ chi, double2* __restrict__ C, int L) {
+  if (*stop) return;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   double2* buf = reinterpret_cast<double2*>(lds_raw);
   const int nz = g.nz, ls = g.lsz;
@@ -76,10 +77,11 @@ __global__ void __launch_bounds__(kPfNT) k_pf_z_fwd(GridDev g, const double* __r
 
 // z lines, inverse, then d = Re / N, chi_new = B - (A - A0) d and the block
 // partials of sum (chi_new - chi)^2 and sum chi_new^2
-__global__ void __launch_bounds__(kPfNT) k_pf_z_inv(GridDev g, const double2* __restrict__ C,
-                                                    const double* __restrict__ H, double inv_lc2, double gclc,
-                                                    double A0, double* __restrict__ chi, double* __restrict__ d,
-                                                    double* __restrict__ part, int L) {
+__global__ void __launch_bounds__(kPfNT) k_pf_z_inv(GridDev g, const int* __restrict__ stop,
+                                                    const double2* __restrict__ C, const double* __restrict__ H,
+                                                    double inv_lc2, double gclc, double A0, double* __restrict__ chi,
+                                                    double* __restrict__ d, double* __restrict__ part, int L) {
+  if (*stop) return;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   double2* buf = reinterpret_cast<double2*>(lds_raw);
   __shared__ double red[kPfNT / 64];
@@ -116,7 +118,9 @@ __global__ void __launch_bounds__(kPfNT) k_pf_z_inv(GridDev g, const double2* __
 
 // y lines of T consecutive z columns at one x
 template <bool INV>
-__global__ void __launch_bounds__(kPfNT) k_pf_y(GridDev g, double2* __restrict__ C, FastDiv f_T, int nchunks) {
+__global__ void __launch_bounds__(kPfNT) k_pf_y(GridDev g, const int* __restrict__ stop, double2* __restrict__ C,
+                                                FastDiv f_T, int nchunks) {
+  if (*stop) return;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   double2* buf = reinterpret_cast<double2*>(lds_raw);
   const int T = (int)f_T.d;
@@ -139,8 +143,9 @@ __global__ void __launch_bounds__(kPfNT) k_pf_y(GridDev g, double2* __restrict__
 
 // x lines of T consecutive (y, z) columns: forward, divide by A0 + |xi~|^2,
 // inverse
-__global__ void __launch_bounds__(kPfNT) k_pf_x(GridDev g, PfTables tb, double2* __restrict__ C, FastDiv f_T,
-                                                double A0) {
+__global__ void __launch_bounds__(kPfNT) k_pf_x(GridDev g, const int* __restrict__ stop, PfTables tb,
+                                                double2* __restrict__ C, FastDiv f_T, double A0) {
+  if (*stop) return;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   double2* buf = reinterpret_cast<double2*>(lds_raw);
   const int T = (int)f_T.d;
@@ -172,8 +177,9 @@ __global__ void __launch_bounds__(kPfNT) k_pf_x(GridDev g, PfTables tb, double2*
 }
 
 // sums of the block partials in a fixed order: out[0..1]
-__global__ void __launch_bounds__(kThreads) k_pf_sum(const double* __restrict__ part, int nblk,
-                                                     double* __restrict__ out) {
+__global__ void __launch_bounds__(kThreads) k_pf_sum(const int* __restrict__ stop, const double* __restrict__ part,
+                                                     int nblk, double* __restrict__ out) {
+  if (*stop) return;
   __shared__ double red[kThreads / 64];
   for (int c = 0; c < 2; ++c) {
     double a = 0.0;
@@ -181,6 +187,22 @@ __global__ void __launch_bounds__(kThreads) k_pf_sum(const double* __restrict__ 
     const double s = block_sum(a, red);
     if (threadIdx.x == 0) out[c] = s;
   }
+}
+
+// Stop test after an iteration, as phase_field_solve's loop condition
+// (lippmann_schwinger.py:525-530, 540-545): res = ||chi_new - chi|| / ||chi_new||, or 0
+// when ||chi_new|| <= 1e-12; state[0] = stop, state[1] = iterations done.
+// Once stop is set the kernels of the remaining iterations of a replayed
+// chunk return at once, so the fields and the count are those of the
+// one-iteration-at-a-time loop.
+__global__ void k_pf_check(const double* __restrict__ sums, double tol, int maxiter, int* __restrict__ state,
+                           double* __restrict__ res) {
+  if (threadIdx.x != 0 || state[0]) return;
+  const double nd = sqrt(sums[0]), nc = sqrt(sums[1]);
+  const double r = nc > 1e-12 ? nd / nc : 0.0;
+  *res = r;
+  state[1] += 1;
+  if (!(r > tol) || state[1] >= maxiter) state[0] = 1;
 }
 
 // min and max of a FP64 field: per block, then one block over the partials

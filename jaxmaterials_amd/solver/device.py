@@ -145,9 +145,12 @@ class LSSolver:
             out = torch.empty_like(d_old)
         it = ctypes.c_int(0)
         res = ctypes.c_double(0.0)
+        # the domain size in FP64, as the reference's FP64 phase field builds xi~
+        ext = np.array([self.gs.Lx, self.gs.Ly, self.gs.Lz], dtype=np.float64)
         torch.cuda.current_stream(history.device).synchronize()
         rc = self.lib.ls_solver_phase_field(self._h, _ptr(history), _ptr(d_old), float(gc), float(lc), float(tolerance),
-                                            int(maxiter), _ptr(out), ctypes.byref(it), ctypes.byref(res))
+                                            int(maxiter), _ptr(out), ctypes.byref(it), ctypes.byref(res),
+                                            ext.ctypes.data_as(ctypes.c_void_p))
         if rc != 0:
             raise RuntimeError("ls_solver_phase_field failed (see stderr)")
         return out, it.value, res.value
