@@ -1104,7 +1104,10 @@ __global__ void __launch_bounds__(kThreads) k_aa_solve(const RhsState* __restric
     }
   }
   for (int s = f; s < m1; ++s) A[s * m1 + s] = 1.0;  // unfilled ages keep the identity
-  double M[kMaxDepth1][kMaxDepth1 + 1];
+  // the elimination works in LDS: a thread-local array indexed at run time
+  // would live in scratch memory, one global-latency access per element
+  __shared__ double M[kMaxDepth1][kMaxDepth1 + 1];
+  __shared__ double v[kMaxDepth1];
   for (int i = 0; i < f; ++i) {
     for (int j = 0; j < f; ++j) M[i][j] = A[i * m1 + j];
     M[i][f] = 1.0;
@@ -1129,7 +1132,6 @@ __global__ void __launch_bounds__(kThreads) k_aa_solve(const RhsState* __restric
       for (int j = c; j <= f; ++j) M[i][j] -= q * M[c][j];
     }
   }
-  double v[kMaxDepth1];
   double sv = 0.0;
   if (ok) {
     for (int i = f - 1; i >= 0; --i) {

@@ -705,7 +705,8 @@ int create_impl(ls_solver* s, const int* voxels, const float* extents, int nrhs,
     s->chunk_g = ((s->dl.per_rhs + s->nblk_g - 1) / s->nblk_g + 1) / 2 * 2;
     // self-conjugate planes: k_z = 0, and k_z = nz/2 for even nz
     s->pl_slot = (long long)(s->nz % 2 == 0 ? 2 : 1) * s->nx * s->ny * 6;
-    s->nblk_p = (int)std::min<long long>(1024, (s->pl_slot + 2047) / 2048);
+    s->nblk_p = (int)std::min<long long>(
+        1024, std::max((s->pl_slot + 2047) / 2048, std::min<long long>(kCUs, (s->pl_slot + 255) / 256)));
     s->chunk_p = (s->pl_slot + s->nblk_p - 1) / s->nblk_p;
     if (dalloc(&s->aaG, hist) || dalloc(&s->aaU, (size_t)nrhs * s->m1 * s->dl.per_rhs) ||
         dalloc(&s->aaPl, (size_t)nrhs * s->m1 * s->pl_slot) || dalloc(&s->aa_alpha, (size_t)nrhs * s->m1) ||
