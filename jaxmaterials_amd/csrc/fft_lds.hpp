@@ -285,6 +285,62 @@ __device__ __forceinline__ void direct_dft(V* buf, const FastDiv& f_n, const Fas
   __syncthreads();
 }
 
+// One Stockham stage of a radix the butterflies above do not cover (a factor
+// of the length with primes above 5, e.g. 7, 11, 13 or a product of them):
+// every work item computes one output of an R-point DFT of a strided
+// subsequence, with one table twiddle per term.  Output q_out of the
+// butterfly (jq, k), j = jq Ns + k, is
+//   sum_q x[j + q nR] w^(q (k + q_out Ns) tws),  w = exp(-+2 pi i / n),
+// tws = n / (Ns R) (the stage twiddle w^(q k tws) and the R-point root
+// w^(q q_out n / R) combined), written to jq Ns R + q_out Ns + k, the place
+// stockham_stage writes it.  Requires ceil(nlines * n / NT) <= E.
+template <bool INV, bool LINE_FAST, bool PAD, int E, int NT, typename V>
+__device__ __forceinline__ void generic_stage(V* buf, int n, int R, const FastDiv& f_ns, const FastDiv& f_n,
+                                              const FastDiv& f_lines, int nlines, int es, int ls,
+                                              const V* __restrict__ tw) {
+  const int tid = threadIdx.x;
+  const int Ns = (int)f_ns.d;
+  const int nR = n / R;
+  const int tws = n / (Ns * R);
+  const int total = nlines * n;
+  V v[E];
+#pragma unroll
+  for (int b = 0; b < E; ++b) {
+    const int w = tid + b * NT;
+    if (w < total) {
+      int l, o;
+      split_item<LINE_FAST>(w, f_lines, f_n, l, o);
+      const int t = fdiv(o, f_ns);
+      const int k = o - t * Ns;
+      const int jq = t / R;
+      const int qo = t - jq * R;
+      const int j = jq * Ns + k;
+      const int step = (k + qo * Ns) * tws;  // < n
+      V acc = CT<V>::mk(0, 0);
+      int idx = 0;
+      for (int q = 0; q < R; ++q) {
+        V x = buf[lds_idx<PAD>(l, j + q * nR, es, ls)];
+        V tq = tw[idx];
+        acc = cadd(acc, INV ? cmulc(x, tq) : cmul(x, tq));
+        idx += step;
+        if (idx >= n) idx -= n;
+      }
+      v[b] = acc;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < E; ++b) {
+    const int w = tid + b * NT;
+    if (w < total) {
+      int l, o;
+      split_item<LINE_FAST>(w, f_lines, f_n, l, o);
+      buf[lds_idx<PAD>(l, o, es, ls)] = v[b];
+    }
+  }
+  __syncthreads();
+}
+
 // Transform all lines in place.  Must be called by every thread of the block.
 // f_lines (LINE_FAST layouts) divides by nlines; lines must be < 2^15 and
 // line lengths < 2^16.
@@ -305,7 +361,10 @@ __device__ __forceinline__ void fft_lines(V* buf, const AxisPlan& p, const FastD
       case 4: LS_STAGE(4); break;
       case 2: LS_STAGE(2); break;
       case 3: LS_STAGE(3); break;
-      default: LS_STAGE(5); break;
+      case 5: LS_STAGE(5); break;
+      default:
+        generic_stage<INV, LINE_FAST, PAD, E, NT>(buf, p.n, R, p.f_ns[s], p.f_n, f_lines, nlines, es, ls, tw);
+        break;
     }
 #undef LS_STAGE
   }

@@ -1093,25 +1093,29 @@ __global__ void __launch_bounds__(kThreads) k_aa_solve(const RhsState* __restric
       if (s < m1) row[s] += di[s];
   }
   block_sum_n(row, m1, red);
-  if (threadIdx.x != 0) return;
-  double* A = gram + (long long)r * m1 * m1;
-#pragma unroll
-  for (int s = 0; s < kMaxDepth1; ++s) {
-    if (s < m1) {
-      const double v = s < f ? row[s] : 0.0;
-      A[s] = v;
-      A[s * m1] = v;
-    }
-  }
-  for (int s = f; s < m1; ++s) A[s * m1 + s] = 1.0;  // unfilled ages keep the identity
-  // the elimination works in LDS: a thread-local array indexed at run time
-  // would live in scratch memory, one global-latency access per element
+  // the elimination works in LDS (a thread-local array indexed at run time
+  // would live in scratch memory); the block updates the stored Gram matrix
+  // (new row and column 0, identity on unfilled ages) and loads the filled
+  // block into LDS in parallel, then one thread eliminates
   __shared__ double M[kMaxDepth1][kMaxDepth1 + 1];
   __shared__ double v[kMaxDepth1];
-  for (int i = 0; i < f; ++i) {
-    for (int j = 0; j < f; ++j) M[i][j] = A[i * m1 + j];
-    M[i][f] = 1.0;
+  __shared__ double row0[kMaxDepth1];
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int s = 0; s < kMaxDepth1; ++s) row0[s] = s < f ? row[s] : 0.0;
   }
+  __syncthreads();
+  double* A = gram + (long long)r * m1 * m1;
+  for (int e = threadIdx.x; e < m1 * m1; e += kThreads) {
+    const int i = e / m1, j = e - i * m1;
+    double a = i == 0 ? row0[j] : (j == 0 ? row0[i] : A[e]);
+    if (i == j && i >= f) a = 1.0;  // unfilled ages keep the identity
+    A[e] = a;
+    if (i < f && j < f) M[i][j] = a;
+  }
+  if (threadIdx.x < f) M[threadIdx.x][f] = 1.0;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
   bool ok = true;
   for (int c = 0; c < f && ok; ++c) {
     int piv = c;

@@ -66,7 +66,10 @@ std::vector<int> factorise(int n) {
     m /= 5;
     p5.push_back(5);
   }
-  if (m != 1) return {};  // prime factor > 5: direct DFT
+  // m > 1: prime factors above 5, transformed by one generic stage
+  // (fft_lds.hpp's generic_stage) after the others; without a factor 2, 3 or
+  // 5 at all the whole line is one direct DFT
+  if (m != 1 && m == n) return {};
   // powers of two as 8s and 4s (16 = 4 x 4), a single 2 only for odd exponents left over
   int e = e2;
   while (e >= 3 && e != 4) {
@@ -80,13 +83,21 @@ std::vector<int> factorise(int n) {
   if (e == 1) r.push_back(2);
   for (int v : p3) r.push_back(v);
   for (int v : p5) r.push_back(v);
+  if (m != 1) r.push_back(m);
   return r;
 }
+
+// radices with register butterflies; any other radix runs generic_stage
+bool butterfly_radix(int R) { return R == 2 || R == 3 || R == 4 || R == 5 || R == 8; }
 
 bool fits(const HostAxis& a, int nlines, int E, int NT) {
   const long long tot = (long long)nlines * a.n;
   if (a.radix.empty()) return (tot + NT - 1) / NT <= E;
   for (int R : a.radix) {
+    if (!butterfly_radix(R)) {
+      if ((tot + NT - 1) / NT > E) return false;  // one output per work item
+      continue;
+    }
     const long long bf = tot / R;
     if (((bf + NT - 1) / NT) * R > E) return false;
   }

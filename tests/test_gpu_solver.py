@@ -245,6 +245,31 @@ def test_register_fft_lengths_match_oracle(gpu, shape):
         s.close()
 
 
+# Axis lengths with a prime factor above 5 next to factors 2, 3 or 5 (the LDS
+# engine's butterfly stages followed by one generic stage), and a length that
+# is a product of such primes only (one direct DFT).
+MIXED_AXES = [(22, 21, 26), (55, 14, 9), (39, 34, 8), (77, 6, 10)]
+
+
+@pytest.mark.parametrize("shape", MIXED_AXES, ids=lambda s: "x".join(map(str, s)))
+def test_mixed_radix_lengths_match_oracle(gpu, shape):
+    """Strain after a fixed number of plain and Anderson updates against the FP64
+    oracle on grids whose axis lengths need the generic FFT stage."""
+    gs = GridSpec(*shape, 1.0, 0.9, 0.3)
+    lmbda, mu = random_material(gs, seed=23, lo=0.5, hi=1.5)
+    lam_t = torch.from_numpy(lmbda).to(gpu)
+    mu_t = torch.from_numpy(mu).to(gpu)
+    for depth in (0, 3):
+        s = LSSolver(gs, nrhs=1, depth=depth)
+        eb = EPS_BAR[None].astype(np.float32)
+        out = s.solve(lam_t, mu_t, eb, rtol=1e-20, atol=0.0, maxiter=6, check=False)
+        e_k, _, _ = oracle_fixed_updates(lmbda, mu, EPS_BAR, gs, 6, depth=depth)
+        err = rel_l2(out["epsilon"][0].cpu().numpy(), e_k)
+        print(f"mixed-radix FFT {shape} depth {depth}: strain relL2 after 6 updates={err:.2e}", flush=True)
+        assert err <= 1e-5
+        s.close()
+
+
 def _register_configs():
     """(kind, n, index) of every compiled register FFT configuration, read from
     the LS_REG_Y_LIST / LS_REG_H_LIST tables of the solver source (the index
