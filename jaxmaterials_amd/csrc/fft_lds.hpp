@@ -294,14 +294,16 @@ __device__ __forceinline__ void direct_dft(V* buf, const FastDiv& f_n, const Fas
 // tws = n / (Ns R) (the stage twiddle w^(q k tws) and the R-point root
 // w^(q q_out n / R) combined), written to jq Ns R + q_out Ns + k, the place
 // stockham_stage writes it.  Requires ceil(nlines * n / NT) <= E.
+// f_nsR divides by Ns R (the next stage's Ns).
 template <bool INV, bool LINE_FAST, bool PAD, int E, int NT, typename V>
-__device__ __forceinline__ void generic_stage(V* buf, int n, int R, const FastDiv& f_ns, const FastDiv& f_n,
-                                              const FastDiv& f_lines, int nlines, int es, int ls,
-                                              const V* __restrict__ tw) {
+__device__ __forceinline__ void generic_stage(V* buf, int n, int R, const FastDiv& f_ns, const FastDiv& f_nsR,
+                                              const FastDiv& f_n, const FastDiv& f_lines, int nlines, int es,
+                                              int ls, const V* __restrict__ tw) {
   const int tid = threadIdx.x;
   const int Ns = (int)f_ns.d;
+  const int NsR = (int)f_nsR.d;
   const int nR = n / R;
-  const int tws = n / (Ns * R);
+  const int tws = n / NsR;
   const int total = nlines * n;
   V v[E];
 #pragma unroll
@@ -310,14 +312,15 @@ __device__ __forceinline__ void generic_stage(V* buf, int n, int R, const FastDi
     if (w < total) {
       int l, o;
       split_item<LINE_FAST>(w, f_lines, f_n, l, o);
-      const int t = fdiv(o, f_ns);
-      const int k = o - t * Ns;
-      const int jq = t / R;
-      const int qo = t - jq * R;
+      const int jq = fdiv(o, f_nsR);
+      const int rem = o - jq * NsR;
+      const int qo = fdiv(rem, f_ns);
+      const int k = rem - qo * Ns;
       const int j = jq * Ns + k;
       const int step = (k + qo * Ns) * tws;  // < n
       V acc = CT<V>::mk(0, 0);
       int idx = 0;
+#pragma unroll 1
       for (int q = 0; q < R; ++q) {
         V x = buf[lds_idx<PAD>(l, j + q * nR, es, ls)];
         V tq = tw[idx];
@@ -343,8 +346,11 @@ __device__ __forceinline__ void generic_stage(V* buf, int n, int R, const FastDi
 
 // Transform all lines in place.  Must be called by every thread of the block.
 // f_lines (LINE_FAST layouts) divides by nlines; lines must be < 2^15 and
-// line lengths < 2^16.
-template <bool INV, bool LINE_FAST, bool PAD, int E, int NT, typename V>
+// line lengths < 2^16.  GEN = false leaves generic_stage out (it raises the
+// register count of the kernel even where it never runs); only for plans
+// without such a stage (the z axis, whose lengths with primes above 5 take
+// the direct DFT).
+template <bool INV, bool LINE_FAST, bool PAD, int E, int NT, bool GEN = true, typename V>
 __device__ __forceinline__ void fft_lines(V* buf, const AxisPlan& p, const FastDiv& f_lines, int nlines, int es,
                                           int ls) {
   const V* tw = CT<V>::tw(p);
@@ -363,7 +369,9 @@ __device__ __forceinline__ void fft_lines(V* buf, const AxisPlan& p, const FastD
       case 3: LS_STAGE(3); break;
       case 5: LS_STAGE(5); break;
       default:
-        generic_stage<INV, LINE_FAST, PAD, E, NT>(buf, p.n, R, p.f_ns[s], p.f_n, f_lines, nlines, es, ls, tw);
+        if constexpr (GEN)
+          generic_stage<INV, LINE_FAST, PAD, E, NT>(buf, p.n, R, p.f_ns[s], p.f_ns[s + 1], p.f_n, f_lines, nlines,
+                                                    es, ls, tw);
         break;
     }
 #undef LS_STAGE

@@ -324,7 +324,7 @@ __global__ void __launch_bounds__(NT) k_zline(GridDev g, const RhsDesc* __restri
       if (!self) lds[lds_idx<true>(l * 3 + p, nz - k, 1, ls)] = make_float2(A.x + B.y, B.x - A.y);
     }
     __syncthreads();
-    fft_lines<true, false, true, E, NT>(lds, g.pz, f_none, 3 * L, 1, ls);
+    fft_lines<true, false, true, E, NT, false>(lds, g.pz, f_none, 3 * L, 1, ls);
   }
 
   // pointwise: update / Hooke, write sigma pairs into LDS
@@ -443,7 +443,7 @@ __global__ void __launch_bounds__(NT) k_zline(GridDev g, const RhsDesc* __restri
     return;
   } else {
     __syncthreads();
-    fft_lines<false, false, true, E, NT>(lds, g.pz, f_none, 3 * L, 1, ls);
+    fft_lines<false, false, true, E, NT, false>(lds, g.pz, f_none, 3 * L, 1, ls);
     __syncthreads();
     store_half_pairs<NT>(lds, g, L, S, (long long)line0);
   }
@@ -679,7 +679,7 @@ __global__ void __launch_bounds__(NT) k_base_z(GridDev g, const RhsDesc* __restr
     for (int p = 0; p < 3; ++p) lds[lds_idx<true>(l * 3 + p, j, 1, ls)] = make_double2(s[2 * p], s[2 * p + 1]);
   }
   __syncthreads();
-  fft_lines<false, false, true, E, NT>(lds, g.pz, g.f_nz, 3 * L, 1, ls);
+  fft_lines<false, false, true, E, NT, false>(lds, g.pz, g.f_nz, 3 * L, 1, ls);
   __syncthreads();
   store_half_pairs<NT>(lds, g, L, spec64, (long long)line0);
 }

@@ -50,7 +50,7 @@ struct HostAxis {
   float2* d_ph = nullptr;  // e^{+i pi k / n}
 };
 
-std::vector<int> factorise(int n) {
+std::vector<int> factorise(int n, bool generic) {
   std::vector<int> r;
   int m = n, e2 = 0;
   while (m % 2 == 0) {
@@ -68,8 +68,9 @@ std::vector<int> factorise(int n) {
   }
   // m > 1: prime factors above 5, transformed by one generic stage
   // (fft_lds.hpp's generic_stage) after the others; without a factor 2, 3 or
-  // 5 at all the whole line is one direct DFT
-  if (m != 1 && m == n) return {};
+  // 5 at all, or on an axis whose kernels leave the generic stage out, the
+  // whole line is one direct DFT
+  if (m != 1 && (m == n || !generic)) return {};
   // powers of two as 8s and 4s (16 = 4 x 4), a single 2 only for odd exponents left over
   int e = e2;
   while (e >= 3 && e != 4) {
@@ -275,9 +276,9 @@ int dalloc(T** p, size_t count) {
   return 0;
 }
 
-int build_axis(HostAxis& a, int n) {
+int build_axis(HostAxis& a, int n, bool generic) {
   a.n = n;
-  a.radix = factorise(n);
+  a.radix = factorise(n, generic);
   std::vector<float2> tw(n);
   std::vector<double2> tw64(n);
   for (int t = 0; t < n; ++t) {
@@ -616,8 +617,9 @@ int create_impl(ls_solver* s, const int* voxels, const float* extents, int nrhs,
     s->own_stream = true;
   }
   const int n[3] = {s->nx, s->ny, s->nz};
+  // the z-axis kernels are compiled without the generic FFT stage
   for (int d = 0; d < 3; ++d)
-    if (build_axis(s->ax[d], n[d])) return -1;
+    if (build_axis(s->ax[d], n[d], d != 2)) return -1;
   // tiles: aim for ~3-6k complex elements per block in y and x, ~1.5k in z
   // (two lines of six components at 256: 2% faster z passes than four), but
   // on small grids no more lines than leave about one block per CU: there a

@@ -68,7 +68,7 @@ __global__ void __launch_bounds__(kPfNT) k_pf_z_fwd(GridDev g, const int* __rest
     buf[lds_idx<true>(l, k, 1, ls)] = make_double2(chi[(long long)(line0 + l) * nz + k], 0.0);
   }
   __syncthreads();
-  fft_lines<false, false, true, kPfE, kPfNT>(buf, g.pz, g.f_nz, lines, 1, ls);
+  fft_lines<false, false, true, kPfE, kPfNT, false>(buf, g.pz, g.f_nz, lines, 1, ls);
   for (int e = threadIdx.x; e < lines * nz; e += kPfNT) {
     const int l = e / nz, k = e - l * nz;
     C[(long long)(line0 + l) * nz + k] = buf[lds_idx<true>(l, k, 1, ls)];
@@ -94,7 +94,7 @@ __global__ void __launch_bounds__(kPfNT) k_pf_z_inv(GridDev g, const int* __rest
     buf[lds_idx<true>(l, k, 1, ls)] = C[(long long)(line0 + l) * nz + k];
   }
   __syncthreads();
-  fft_lines<true, false, true, kPfE, kPfNT>(buf, g.pz, g.f_nz, lines, 1, ls);
+  fft_lines<true, false, true, kPfE, kPfNT, false>(buf, g.pz, g.f_nz, lines, 1, ls);
   const double invN = 1.0 / (double)g.N;
   double s_diff = 0.0, s_new = 0.0;
   for (int e = threadIdx.x; e < lines * nz; e += kPfNT) {
