@@ -638,7 +638,12 @@ __global__ void __launch_bounds__(NT) k_xpass(GridDev g, const RhsDesc* __restri
 // ---------------------------------------------------------------------------
 // FP64 base evaluation of one load case r (spec64 holds that load case only).
 // z lines: sigma = C base + tau in FP64, R2C along z in FP64.
-template <int E, int NT>
+// LM: the uniform base eps_bar of every load case of one material instead:
+// lambda and mu of load case r's material in the slots of sigma_0 and
+// sigma_1 (the other four zero), since then
+// sigma_hat = tr(eps_bar) lambda_hat I + 2 mu_hat eps_bar for each of them
+// (k_base_x<LM> forms their divergences).
+template <int E, int NT, bool LM = false>
 __global__ void __launch_bounds__(NT) k_base_z(GridDev g, const RhsDesc* __restrict__ rhs, int r,
                                                const float* __restrict__ lam, const float* __restrict__ mu,
                                                const float* __restrict__ base, const float* __restrict__ tau,
@@ -661,6 +666,15 @@ __global__ void __launch_bounds__(NT) k_base_z(GridDev g, const RhsDesc* __restr
     const int l = fdiv(e, g.f_nz), j = e - l * nz;
     const long long v = (long long)(line0 + l) * nz + j;
     double ev[6], s[6];
+    if constexpr (LM) {
+      s[0] = LA[v];
+      s[1] = MU[v];
+#pragma unroll
+      for (int c = 2; c < 6; ++c) s[c] = 0.0;
+#pragma unroll
+      for (int p = 0; p < 3; ++p) lds[lds_idx<true>(l * 3 + p, j, 1, ls)] = make_double2(s[2 * p], s[2 * p + 1]);
+      continue;
+    }
 #pragma unroll
     for (int c = 0; c < 6; ++c) ev[c] = B_[c * N + v];
     hooke(ev, (double)LA[v], (double)MU[v], s);
@@ -686,10 +700,14 @@ __global__ void __launch_bounds__(NT) k_base_z(GridDev g, const RhsDesc* __restr
 
 // x lines: forward FFT along x in FP64, d_base = sigma_hat . xi~ (the FP32
 // tables of the iteration, promoted, so both use the same xi~), base DC.
-template <int E, int NT>
+// LM (after k_base_z<LM>): spec64 holds lambda_hat, mu_hat; d_base and the
+// base DC of the load cases r .. r + nr - 1 with sigma_hat =
+// tr(eps_bar) lambda_hat I + 2 mu_hat eps_bar.
+template <int E, int NT, bool LM = false>
 __global__ void __launch_bounds__(NT) k_base_x(GridDev g, int r, const double2* __restrict__ spec64,
                                                float2* __restrict__ dbase, RhsState* __restrict__ st, FastDiv f_Tm,
-                                               FastDiv f_cols, DbLayout dl) {
+                                               FastDiv f_cols, DbLayout dl, const RhsDesc* __restrict__ rhs = nullptr,
+                                               int nr = 1) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const int Tm = (int)f_Tm.d;
   double2* lds = reinterpret_cast<double2*>(lds_raw);
@@ -706,8 +724,45 @@ __global__ void __launch_bounds__(NT) k_base_x(GridDev g, int r, const double2* 
   }
   __syncthreads();
   fft_lines<false, true, false, E, NT>(lds, g.px, f_cols, cols, cols, 1);
-  float2* DB = dbase + (long long)r * dl.per_rhs;
   const int nmodes = nx * Tm;
+  if constexpr (LM) {
+    for (int e = threadIdx.x; e < nmodes; e += NT) {
+      const int x = fdiv(e, f_Tm), m = e - x * Tm;
+      const int gm = m0 + m;
+      if (gm >= nm) continue;
+      const int ky = fdiv(gm, g.f_P), kz = gm - ky * P;
+      const double2 la = lds[x * cols + 6 * m], mu = lds[x * cols + 6 * m + 1];
+      float f0, f1, f2;
+      xi_tilde(g, x, ky, kz, f0, f1, f2);
+      const double k[3] = {f0, f1, f2};
+      const int cs = dl.cstride();
+      for (int q = r; q < r + nr; ++q) {
+        double eb[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) eb[c] = rhs[q].eps_bar[c];
+        const double tr = eb[0] + eb[1] + eb[2];
+        // sigma_hat_c = (c < 3 ? tr lambda_hat : 0) + 2 mu_hat eps_bar_c
+        double2 sg[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+          sg[c] = make_double2((c < 3 ? tr * la.x : 0.0) + 2.0 * eb[c] * mu.x,
+                               (c < 3 ? tr * la.y : 0.0) + 2.0 * eb[c] * mu.y);
+        if (x == 0 && gm == 0) {
+#pragma unroll
+          for (int c = 0; c < 6; ++c) st[q].dcb[c] = sg[c].x;
+        }
+        float2* o = dbase + (long long)q * dl.per_rhs + dl.index(x, gm);
+        o[0] = make_float2((float)(k[0] * sg[0].x + k[1] * sg[3].x + k[2] * sg[4].x),
+                           (float)(k[0] * sg[0].y + k[1] * sg[3].y + k[2] * sg[4].y));
+        o[cs] = make_float2((float)(k[0] * sg[3].x + k[1] * sg[1].x + k[2] * sg[5].x),
+                            (float)(k[0] * sg[3].y + k[1] * sg[1].y + k[2] * sg[5].y));
+        o[2 * cs] = make_float2((float)(k[0] * sg[4].x + k[1] * sg[5].x + k[2] * sg[2].x),
+                                (float)(k[0] * sg[4].y + k[1] * sg[5].y + k[2] * sg[2].y));
+      }
+    }
+    return;
+  }
+  float2* DB = dbase + (long long)r * dl.per_rhs;
   for (int e = threadIdx.x; e < nmodes; e += NT) {
     const int x = fdiv(e, f_Tm), m = e - x * Tm;
     const int gm = m0 + m;

@@ -234,6 +234,7 @@ struct ls_solver {
   int nblk_stress = 0, chunk_stress = 0;
   int nblk_mm = 0, chunk_mm = 0;
   double rebase_factor = 1e-4;  // LS_REBASE_FACTOR; one rebase per 256^3 solve to 1e-6
+  std::vector<int> h_mat;       // material of each load case (current solve)
   int n_rebase = 0;
   int absolute = 0;  // LS_ABSOLUTE_RESIDUAL flag of the current solve
   // Anderson
@@ -523,6 +524,48 @@ int put_sel(ls_solver* s, const std::vector<int>& sel) {
 void base_eval_active(ls_solver* s, const float* lam, const float* mu, const float* tau) {
   for (int r = 0; r < s->nrhs; ++r)
     if (s->h_st[r].active) base_eval_one(s, r, lam, mu, tau);
+}
+
+// The same for bases that are the uniform eps_bar of every active load case
+// (the start of a solve) with Hooke's law and no polarisation: one FP64
+// transform of lambda and mu per run of consecutive active load cases of one
+// material, sigma_hat = tr(eps_bar) lambda_hat I + 2 mu_hat eps_bar
+// (k_base_z<.., true>, k_base_x<.., true>), instead of six stress transforms
+// per load case.  Otherwise the per-load-case evaluation.
+void base_eval_start(ls_solver* s, const float* lam, const float* mu, const float* tau) {
+  if (tau || s->gd.dmg || (int)s->h_mat.size() != s->nrhs) {
+    base_eval_active(s, lam, mu, tau);
+    return;
+  }
+  int r = 0;
+  while (r < s->nrhs) {
+    if (!s->h_st[r].active) {
+      ++r;
+      continue;
+    }
+    int q = r + 1;
+    while (q < s->nrhs && s->h_st[q].active && s->h_mat[q] == s->h_mat[r]) ++q;
+#define LS_BZL(EV, NTV)                                                                                    \
+  k_base_z<EV, NTV, true><<<s->nblk_z64, NTV, s->tz64.lds, s->stream>>>(s->gd, s->d_rhs, r, lam, mu, s->base, \
+                                                                        nullptr, s->spec64, s->tz64.lines)
+    LS_DISPATCH64(s->tz64, LS_BZL)
+#undef LS_BZL
+    const FastDiv fT = make_fastdiv(s->ty64.lines);
+    const AAPlanes nopl = {nullptr, nullptr, 0, 1};
+#define LS_BY(EV, NTV)                                                                                        \
+  k_ypass<false, EV, NTV, double2><<<s->nblk_y64, NTV, s->ty64.lds, s->stream>>>(s->gd, s->d_st, s->spec64,   \
+                                                                                fT, s->ntiles_y64, r, nopl)
+    LS_DISPATCH64(s->ty64, LS_BY)
+#undef LS_BY
+    const FastDiv fTm = make_fastdiv(s->tx64.lines), fcols = make_fastdiv(6 * s->tx64.lines);
+#define LS_BXL(EV, NTV)                                                                                        \
+  k_base_x<EV, NTV, true><<<s->nblk_x64, NTV, s->tx64.lds, s->stream>>>(s->gd, r, s->spec64, s->dbase, s->d_st, \
+                                                                        fTm, fcols, s->dl, s->d_rhs, q - r)
+    LS_DISPATCH64(s->tx64, LS_BXL)
+#undef LS_BXL
+    r = q;
+  }
+  mark(s, 3);
 }
 
 int zero_spec(ls_solver* s, const std::vector<int>& which) {
@@ -921,6 +964,8 @@ int solve_impl(ls_solver* s, const float* d_lambda, const float* d_mu, const int
     d.mu0 = mu0[m];
   }
   LS_HIP(hipMemcpyAsync(s->d_rhs, desc.data(), s->nrhs * sizeof(RhsDesc), hipMemcpyHostToDevice, s->stream));
+  s->h_mat.resize(s->nrhs);
+  for (int r = 0; r < s->nrhs; ++r) s->h_mat[r] = desc[r].mat;
   for (int r = 0; r < s->nrhs; ++r) {
     RhsState& q = s->h_st[r];
     memset(&q, 0, sizeof(q));
@@ -939,7 +984,7 @@ int solve_impl(ls_solver* s, const float* d_lambda, const float* d_mu, const int
     // 1. residual of the uniform start, the normaliser of the tolerance
     if (!(rel_ref > 0.0)) {
       init_fields(s, nullptr, nullptr, nullptr);
-      base_eval_active(s, d_lambda, d_mu, d_tau);
+      base_eval_start(s, d_lambda, d_mu, d_tau);
       if (zero_spec(s, all)) return -1;
       xpass(s, 0);
       check(s, drtol, datol, 1);
@@ -985,7 +1030,7 @@ int solve_impl(ls_solver* s, const float* d_lambda, const float* d_mu, const int
       }
       if (put_state(s) || put_sel(s, restart)) return -1;
       init_fields(s, s->d_sel, d_eps, nullptr);
-      base_eval_active(s, d_lambda, d_mu, d_tau);
+      base_eval_start(s, d_lambda, d_mu, d_tau);
       if (zero_spec(s, restart)) return -1;
       eval_x(s, drtol, datol, 0, false);
       if (sync_state(s)) return -1;
@@ -995,7 +1040,7 @@ int solve_impl(ls_solver* s, const float* d_lambda, const float* d_mu, const int
     }
   } else {
     init_fields(s, nullptr, d_eps, nullptr);
-    base_eval_active(s, d_lambda, d_mu, d_tau);
+    base_eval_start(s, d_lambda, d_mu, d_tau);
     if (zero_spec(s, all)) return -1;
     eval_x(s, drtol, datol, 0, true);
   }
