@@ -680,7 +680,7 @@ int create_impl(ls_solver* s, const int* voxels, const float* extents, int nrhs,
                     choose_tile(s->ax[0], wx, 6, (size_t)6 * s->nx * sizeof(float2), kBudget32, s->tx);
   const bool ok64 =
       choose_tile(s->ax[2], std::max(1, 2048 / (3 * s->nz)), 3, (size_t)3 * lsz * sizeof(double2), kBudget64, s->tz64) &&
-      choose_tile(s->ax[1], std::max(1, std::min(6 * s->P, 2048 / s->ny)), 1, (size_t)s->ny * sizeof(double2), kBudget64,
+      choose_tile(s->ax[1], std::max(1, std::min(6 * s->P, 3584 / s->ny)), 1, (size_t)s->ny * sizeof(double2), kBudget64,
                   s->ty64) &&
       choose_tile(s->ax[0], std::max(1, 2048 / (6 * s->nx)), 6, (size_t)6 * s->nx * sizeof(double2), kBudget64, s->tx64);
   if (!ok32 || !ok64) {
