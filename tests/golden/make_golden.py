@@ -1,7 +1,8 @@
 """Write the golden fixtures of tests/golden/ from the FP64 oracle.
 
-    python tests/golden/make_golden.py small    # config-1 (32^3) fixture, seconds
-    python tests/golden/make_golden.py big      # config-2 (256^3) C_eff, ~2 h on 8 cores (resumable)
+    python tests/golden/make_golden.py small    # 32^3 cross-ply fixture (4 layers, r 2.5, vf 0.4), seconds
+    python tests/golden/make_golden.py big      # 256^3 cross-ply C_eff (16 layers, r 6, vf 0.5), ~2 h on 8 cores
+                                                #   (resumable)
 
 The microstructures come from the seeded cross-ply generator (RSA placement,
 splitmix64), restated in oracle/microstructure_oracle.py; the fields are not
