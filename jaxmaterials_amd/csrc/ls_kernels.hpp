@@ -186,6 +186,27 @@ __device__ __forceinline__ void aa_y(float wk, float q2, float inv, float n0, fl
 
 __device__ __forceinline__ float4 ld4(const float2* p) { return *reinterpret_cast<const float4*>(p); }
 
+// Streaming accesses with the non-temporal hint, used by the z passes for the
+// data they touch once (spectrum records, strain and ring fields): on MI355X a
+// plain grid-stride float4 copy moves 4.9 TB/s, 5.2 TB/s with non-temporal
+// loads and stores, and the Anderson z pass 4.83 -> 5.2 TB/s.  Not in the y
+// and x passes: their narrow column accesses share cache lines between work
+// items, and the hint halved the register y pass (4.7 -> 2.5 TB/s).
+typedef float f2v_t __attribute__((ext_vector_type(2)));
+typedef float f4v_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float ldnt(const float* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ void stnt(float* p, float a) { __builtin_nontemporal_store(a, p); }
+__device__ __forceinline__ void stnt(float2* p, float2 a) {
+  const f2v_t v = {a.x, a.y};
+  __builtin_nontemporal_store(v, reinterpret_cast<f2v_t*>(p));
+}
+__device__ __forceinline__ float4 ld4nt(const void* p) {
+  const f4v_t v = __builtin_nontemporal_load(reinterpret_cast<const f4v_t*>(p));
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+// the rare FP64 passes keep plain accesses
+__device__ __forceinline__ void stnt(double2* p, double2 a) { *p = a; }
+
 // Hooke's law, isotropic: sigma_i = 2 mu eps_i + lambda tr(eps) [i<3]
 // (src/jaxmaterials/solver/lippmann_schwinger.py:20-33).
 template <typename T>
@@ -271,8 +292,8 @@ __device__ __forceinline__ void store_half_pairs(const V* lds, const GridDev& g,
     const V zk = lds[lds_idx<true>(l * 3 + p, k, 1, ls)];
     const V zm = lds[lds_idx<true>(l * 3 + p, k == 0 ? 0 : nz - k, 1, ls)];
     V* o = S + ((line0 + l) * P + k) * 6 + 2 * p;
-    o[0] = CT<V>::mk(Sc(0.5) * (zk.x + zm.x), Sc(0.5) * (zk.y - zm.y));
-    o[1] = CT<V>::mk(Sc(0.5) * (zk.y + zm.y), Sc(-0.5) * (zk.x - zm.x));
+    stnt(o, CT<V>::mk(Sc(0.5) * (zk.x + zm.x), Sc(0.5) * (zk.y - zm.y)));
+    stnt(o + 1, CT<V>::mk(Sc(0.5) * (zk.y + zm.y), Sc(-0.5) * (zk.x - zm.x)));
   }
 }
 
@@ -313,7 +334,7 @@ __global__ void __launch_bounds__(NT) k_zline(GridDev g, const RhsDesc* __restri
       const int p = e - 3 * e3;
       const int l = fdiv(e3, g.f_P);
       const int k = e3 - l * P;
-      float4 ab = ld4(S + ((long long)(line0 + l) * P + k) * 6 + 2 * p);
+      float4 ab = ld4nt(S + ((long long)(line0 + l) * P + k) * 6 + 2 * p);
       float2 A = make_float2(ab.x, ab.y), B = make_float2(ab.z, ab.w);
       const bool self = (k == 0) || (2 * k == nz);
       if (self) {
@@ -357,7 +378,7 @@ __global__ void __launch_bounds__(NT) k_zline(GridDev g, const RhsDesc* __restri
         if (aph != 0.0) {
           const float a = (float)aph;
 #pragma unroll
-          for (int c = 0; c < 6; ++c) dk[c] = a * Gh[c * N + v];
+          for (int c = 0; c < 6; ++c) dk[c] = a * ldnt(&Gh[c * N + v]);
         } else {
 #pragma unroll
           for (int c = 0; c < 6; ++c) dk[c] = 0.f;
@@ -371,7 +392,7 @@ __global__ void __launch_bounds__(NT) k_zline(GridDev g, const RhsDesc* __restri
           const float* Gs = Gb + sl * slot_stride;
 #pragma unroll
           for (int c = 0; c < 6; ++c) {
-            const float gv = Gs[c * N + v];
+            const float gv = ldnt(&Gs[c * N + v]);
             dn[c] = fmaf(a, gv, dn[c]);
             dk[c] = fmaf(b, gv, dk[c]);
           }
@@ -386,7 +407,7 @@ __global__ void __launch_bounds__(NT) k_zline(GridDev g, const RhsDesc* __restri
         const float a0 = (float)al[h];
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
-          Gh[c * N + v] = gn[c];
+          stnt(&Gh[c * N + v], gn[c]);
           ev[c] = fmaf(a0, gn[c], dn[c]);
         }
       } else {
@@ -399,13 +420,13 @@ __global__ void __launch_bounds__(NT) k_zline(GridDev g, const RhsDesc* __restri
         }
         if constexpr (MODE == Z_UPD) {
 #pragma unroll
-          for (int c = 0; c < 6; ++c) ev[c] = E_[c * N + v] + rr[c];
+          for (int c = 0; c < 6; ++c) ev[c] = ldnt(&E_[c * N + v]) + rr[c];
         } else {
 #pragma unroll
           for (int c = 0; c < 6; ++c) ev[c] = d.eps_bar[c] - rr[c];
         }
 #pragma unroll
-        for (int c = 0; c < 6; ++c) E_[c * N + v] = ev[c];
+        for (int c = 0; c < 6; ++c) stnt(&E_[c * N + v], ev[c]);
       }
       if constexpr (MODE != Z_SET) {
         float s[6];
