@@ -198,8 +198,9 @@ int ls_strain_energy(const int *voxels, const float *d_lambda, const float *d_mu
                      void *stream);
 
 /* Measured HBM streaming ceilings of the current device (a measurement aid,
- * no counterpart in the reference): grid-stride float4 kernels over
- * `bytes`-sized arrays (three are allocated), best of `reps` runs each;
+ * no counterpart in the reference): grid-stride float4 kernels with
+ * non-temporal loads and stores over `bytes`-sized arrays (three are
+ * allocated), best of `reps` runs each;
  * gbps[4] = GB/s of copy (read + write counted), read only, write only, and
  * two reads + one write.  0 on success, -1 on error. */
 int ls_hbm_stream_probe(long long bytes, int reps, double *gbps, void *stream);
