@@ -1093,13 +1093,13 @@ __global__ void __launch_bounds__(kThreads) k_aa_gram(const RhsState* __restrict
 #pragma unroll
   for (int q = 0; q < kMaxDepth1; ++q) acc[q] = 0.0;
   for (long long i = i0 + threadIdx.x; i < i1; i += kThreads) {
-    const float4 a = yn[i];
+    const float4 a = ld4nt(&yn[i]);
 #pragma unroll
     for (int q = 0; q < kMaxDepth1; ++q) {
       if (q < f) {
         int sl = hd - q;
         if (sl < 0) sl += aa.m1;
-        const float4 b = q == 0 ? a : base[sl * slot4 + i];
+        const float4 b = q == 0 ? a : ld4nt(&base[sl * slot4 + i]);
         acc[q] += (double)a.x * b.x + (double)a.y * b.y + (double)a.z * b.z + (double)a.w * b.w;
       }
     }

@@ -757,7 +757,7 @@ int create_impl(ls_solver* s, const int* voxels, const float* extents, int nrhs,
     // 16k complex per block on large grids; on small ones about 512 per block
     // (up to one block per CU), so the latency-bound row finishes sooner
     s->nblk_g = (int)std::min<long long>(
-        512, std::max((s->dl.per_rhs + 16383) / 16384, std::min<long long>(kCUs, (s->dl.per_rhs + 511) / 512)));
+        2048, std::max((s->dl.per_rhs + 16383) / 16384, std::min<long long>(kCUs, (s->dl.per_rhs + 511) / 512)));
     s->chunk_g = ((s->dl.per_rhs + s->nblk_g - 1) / s->nblk_g + 1) / 2 * 2;
     // self-conjugate planes: k_z = 0, and k_z = nz/2 for even nz
     s->pl_slot = (long long)(s->nz % 2 == 0 ? 2 : 1) * s->nx * s->ny * 6;
