@@ -124,7 +124,7 @@ AxisPlan to_dev(const HostAxis& a) {
 
 constexpr size_t kMaxLds = 60 * 1024;
 constexpr int kMaxTm = 256;  // largest x-pass tile (modes per block); bounds the d_base padding
-constexpr int kCUs = 256;     // compute units of an MI355X: the default tiles leave one block per CU on small grids
+constexpr int kCUs = 256;     // compute units of an MI355X (fallback when the device query fails)
 
 struct Tile {
   int lines = 0;  // Lz (z), T (y) or Tm (x)
@@ -234,6 +234,7 @@ struct ls_solver {
   int nblk_stress = 0, chunk_stress = 0;
   int nblk_mm = 0, chunk_mm = 0;
   double rebase_factor = 1e-4;  // LS_REBASE_FACTOR; one rebase per 256^3 solve to 1e-6
+  int ncu = kCUs;               // compute units of the device: small grids get about one block per CU
   std::vector<int> h_mat;       // material of each load case (current solve)
   int n_rebase = 0;
   int absolute = 0;  // LS_ABSOLUTE_RESIDUAL flag of the current solve
@@ -659,6 +660,12 @@ int create_impl(ls_solver* s, const int* voxels, const float* extents, int nrhs,
     LS_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     s->own_stream = true;
   }
+  {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
+      s->ncu = cus;
+  }
   const int n[3] = {s->nx, s->ny, s->nz};
   // the z-axis kernels are compiled without the generic FFT stage
   for (int d = 0; d < 3; ++d)
@@ -668,8 +675,9 @@ int create_impl(ls_solver* s, const int* voxels, const float* extents, int nrhs,
   // on small grids no more lines than leave about one block per CU: there a
   // pass is latency-bound and more, smaller blocks finish sooner
   // (overridable for tuning)
-  auto per_cu = [](int w, long long units) {
-    return std::max(1, (int)std::min<long long>(w, std::max(1LL, units / kCUs)));
+  const int ncu = s->ncu;
+  auto per_cu = [ncu](int w, long long units) {
+    return std::max(1, (int)std::min<long long>(w, std::max(1LL, units / ncu)));
   };
   const int wz = env_int("LS_TILE_Z", per_cu(std::max(1, 1536 / (3 * s->nz)), (long long)s->nx * s->ny));
   const int wy = env_int("LS_TILE_Y", per_cu(std::max(1, std::min(6 * s->P, 4096 / s->ny)), 6LL * s->nx * s->P));
@@ -757,12 +765,12 @@ int create_impl(ls_solver* s, const int* voxels, const float* extents, int nrhs,
     // 16k complex per block on large grids; on small ones about 512 per block
     // (up to one block per CU), so the latency-bound row finishes sooner
     s->nblk_g = (int)std::min<long long>(
-        2048, std::max((s->dl.per_rhs + 16383) / 16384, std::min<long long>(kCUs, (s->dl.per_rhs + 511) / 512)));
+        2048, std::max((s->dl.per_rhs + 16383) / 16384, std::min<long long>(s->ncu, (s->dl.per_rhs + 511) / 512)));
     s->chunk_g = ((s->dl.per_rhs + s->nblk_g - 1) / s->nblk_g + 1) / 2 * 2;
     // self-conjugate planes: k_z = 0, and k_z = nz/2 for even nz
     s->pl_slot = (long long)(s->nz % 2 == 0 ? 2 : 1) * s->nx * s->ny * 6;
     s->nblk_p = (int)std::min<long long>(
-        1024, std::max((s->pl_slot + 2047) / 2048, std::min<long long>(kCUs, (s->pl_slot + 255) / 256)));
+        1024, std::max((s->pl_slot + 2047) / 2048, std::min<long long>(s->ncu, (s->pl_slot + 255) / 256)));
     s->chunk_p = (s->pl_slot + s->nblk_p - 1) / s->nblk_p;
     if (dalloc(&s->aaG, hist) || dalloc(&s->aaU, (size_t)nrhs * s->m1 * s->dl.per_rhs) ||
         dalloc(&s->aaPl, (size_t)nrhs * s->m1 * s->pl_slot) || dalloc(&s->aa_alpha, (size_t)nrhs * s->m1) ||
@@ -1316,14 +1324,14 @@ int ls_solver_phase_field(ls_solver* s, const double* d_H, const double* d_dold,
   hipStream_t st = s->stream;
   // tiles of the five passes (lines x length <= kPfItems per block; on small
   // grids no more lines than leave about one block per CU)
-  const int L = std::max(1, std::min({1024, kPfItems / nz, nx * ny / kCUs}));
+  const int L = std::max(1, std::min({1024, kPfItems / nz, nx * ny / s->ncu}));
   const int nblk_z = (nx * ny + L - 1) / L;
   const size_t lds_z = (size_t)L * s->gd.lsz * sizeof(double2);
-  const int Ty = std::max(1, std::min({nz, kPfItems / ny, nx * nz / kCUs}));
+  const int Ty = std::max(1, std::min({nz, kPfItems / ny, nx * nz / s->ncu}));
   const int nch_y = (nz + Ty - 1) / Ty;
   const size_t lds_y = (size_t)ny * Ty * sizeof(double2);
   const long long Q = (long long)ny * nz;
-  const int Tx = (int)std::max(1LL, std::min({Q, (long long)(kPfItems / nx), Q / kCUs}));
+  const int Tx = (int)std::max(1LL, std::min({Q, (long long)(kPfItems / nx), Q / s->ncu}));
   const long long nblk_x = (Q + Tx - 1) / Tx;
   const size_t lds_x = (size_t)nx * Tx * sizeof(double2);
   const int nb_mm = (int)std::min<long long>(1024, (N + kThreads - 1) / kThreads);
