@@ -360,7 +360,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--grid", type=int, default=256)
     ap.add_argument("--depth", type=int, default=6)
-    ap.add_argument("--methods", default="plain,aa,aa_ws")
+    ap.add_argument("--methods", default="plain,aa,aa5,aa_ws",
+                    help="plain: no Anderson; aa: Anderson of --depth; aaK: Anderson of depth K (aa5 is the "
+                         "configuration's m = 5); a suffix _ws starts from the learned warm start")
     ap.add_argument("--rtol", type=float, default=1e-6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", choices=("crossply256", "dataset", "phasefield"), default="crossply256",
@@ -408,8 +410,16 @@ def main():
     results = {}
     net = None
     for method in args.methods.split(","):
-        depth = 0 if method.startswith("plain") else args.depth
         warm = method.endswith("_ws")
+        base_name = method[:-3] if warm else method
+        if base_name == "plain":
+            depth = 0
+        elif base_name == "aa":
+            depth = args.depth
+        elif base_name.startswith("aa") and base_name[2:].isdigit():
+            depth = int(base_name[2:])
+        else:
+            raise SystemExit(f"unknown method {method}")
         solver = LSSolver(gs, nrhs=6, depth=depth)
         loads = unit_load_cases()
         eps = torch.empty((6, 6) + shape, dtype=torch.float32, device="cuda")

@@ -1,21 +1,21 @@
-"""Common definitions (mirror of src/jaxmaterials/common.py:1-29)."""
+"""Grid description shared by the solver entry points.
+
+Same class name, constructor arguments and attributes as the reference's
+``GridSpec`` (src/jaxmaterials/common.py:6-29), so reference scripts that
+build one pass it to this package unchanged.
+"""
 
 __all__ = ["GridSpec"]
 
 
 class GridSpec:
-    """Simple class for describing a structured grid"""
+    """A periodic box of nx x ny x nz voxels with side lengths Lx, Ly, Lz
+    (voxel i along x spans [i Lx/nx, (i+1) Lx/nx))."""
 
     def __init__(self, nx, ny, nz, Lx, Ly, Lz):
-        """Initialise instance
-
-        :arg nx: number of voxels in x-direction
-        :arg ny: number of voxels in y-direction
-        :arg nz: number of voxels in z-direction
-        :arg Lx: domain size in x-direction
-        :arg Ly: domain size in y-direction
-        :arg Lz: domain size in z-direction
-        """
+        """Store the voxel counts (nx, ny, nz) and the box lengths (Lx, Ly, Lz)
+        along the three axes; fields on this grid are C-ordered arrays of
+        shape (nx, ny, nz), z varying fastest."""
         self.nx = nx
         self.ny = ny
         self.nz = nz
@@ -25,5 +25,5 @@ class GridSpec:
 
     @property
     def number_of_voxels(self):
-        """Return total number of voxels"""
+        """nx * ny * nz."""
         return self.nx * self.ny * self.nz

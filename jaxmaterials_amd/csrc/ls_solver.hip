@@ -948,7 +948,8 @@ int solve_impl(ls_solver* s, const float* d_lambda, const float* d_mu, const int
     }
     lam0[m] = (float)(0.5 * (b + a));
     mu0[m] = (float)(0.5 * (d + c));
-    if (!(mu0[m] > 0.f) || !(lam0[m] + 2.f * mu0[m] > 0.f) || !std::isfinite(lam0[m])) {
+    if (!(mu0[m] > 0.f) || !(lam0[m] + 2.f * mu0[m] > 0.f) || !std::isfinite(lam0[m]) ||
+        !std::isfinite(mu0[m])) {
       fprintf(stderr, "liblippmannschwinger: invalid reference medium lambda0=%g mu0=%g (material %d)\n",
               lam0[m], mu0[m], m);
       return -1;
@@ -1154,7 +1155,7 @@ int solve_impl(ls_solver* s, const float* d_lambda, const float* d_mu, const int
     if (residual) residual[r] = q.R0 > 0.0 ? q.R / q.R0 : q.R;
     if (sigma_mean)
       for (int c = 0; c < 6; ++c) sigma_mean[r * 6 + c] = s->h_mean6[r * 6 + c];
-    if (q.status != 0 && verbose > 0) fprintf(stderr, "liblippmannschwinger: non-finite residual (rhs %d)\n", r);
+    if (q.status != 0) fprintf(stderr, "liblippmannschwinger: non-finite residual (rhs %d)\n", r);
   }
   if (verbose > 0) {
     const RhsState& q = s->h_st[0];
@@ -1217,6 +1218,13 @@ float time_launches(ls_solver* s, int reps, F&& launch) {
 // the fastest.  The z tile is timed with the plain z-line pass; the Anderson
 // z-line passes share its LDS and register layout.
 int autotune_impl(ls_solver* s, const float* lam, const float* mu, float* eps, int reps, std::string& rep) {
+  // a captured iteration chunk holds the launch shapes and the d_base layout
+  // of the current tiles, which the sweep changes: capture again at the next
+  // graph solve
+  if (s->gexec) {
+    LS_HIP(hipGraphExecDestroy(s->gexec));
+    s->gexec = nullptr;
+  }
   const bool timing = s->timing;
   s->timing = false;
   std::vector<RhsDesc> desc(s->nrhs);
@@ -1324,7 +1332,9 @@ int ls_solver_phase_field(ls_solver* s, const double* d_H, const double* d_dold,
   hipStream_t st = s->stream;
   // tiles of the five passes (lines x length <= kPfItems per block; on small
   // grids no more lines than leave about one block per CU)
-  const int L = std::max(1, std::min({1024, kPfItems / nz, nx * ny / s->ncu}));
+  // (the z tile is bounded by its padded LDS stride, so L lines take at most
+  // kPfItems complex numbers of LDS, like the y and x tiles)
+  const int L = std::max(1, std::min({1024, kPfItems / s->gd.lsz, nx * ny / s->ncu}));
   const int nblk_z = (nx * ny + L - 1) / L;
   const size_t lds_z = (size_t)L * s->gd.lsz * sizeof(double2);
   const int Ty = std::max(1, std::min({nz, kPfItems / ny, nx * nz / s->ncu}));
@@ -1334,6 +1344,16 @@ int ls_solver_phase_field(ls_solver* s, const double* d_H, const double* d_dold,
   const int Tx = (int)std::max(1LL, std::min({Q, (long long)(kPfItems / nx), Q / s->ncu}));
   const long long nblk_x = (Q + Tx - 1) / Tx;
   const size_t lds_x = (size_t)nx * Tx * sizeof(double2);
+  {
+    int dev = 0, lds_max = 0;
+    LS_HIP(hipGetDevice(&dev));
+    LS_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    if (std::max({lds_z, lds_y, lds_x}) > (size_t)lds_max) {
+      fprintf(stderr, "liblippmannschwinger: ls_solver_phase_field: tile needs %zu bytes of LDS, the device has %d\n",
+              std::max({lds_z, lds_y, lds_x}), lds_max);
+      return -1;
+    }
+  }
   const int nb_mm = (int)std::min<long long>(1024, (N + kThreads - 1) / kThreads);
   // FP64 symbol tables: sin / cos (pi k / n) with the exact zeros, 2 n / L
   std::vector<double> tab(2 * (size_t)(nx + ny + nz));

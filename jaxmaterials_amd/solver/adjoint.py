@@ -55,6 +55,8 @@ class LSSolve(torch.autograd.Function):
         ctx.solver = solver
         ctx.maxiter = maxiter
         ctx.adjoint_rtol = adjoint_rtol
+        # the eps_bar gradient is returned on eps_bar's own device and dtype
+        ctx.eb_device, ctx.eb_dtype = eps_bar.device, eps_bar.dtype
         ctx.save_for_backward(lmbda, mu, eps)
         return eps
 
@@ -74,7 +76,8 @@ class LSSolve(torch.autograd.Function):
         d_lam = -(tr_q * tr_e).sum(0)
         d_mu = -2.0 * (q * eps * w).sum(dim=(0, 1))
         Cq = hooke_torch(lmbda.unsqueeze(0), mu.unsqueeze(0), q)
-        d_eb = ((g - Cq).sum(dim=(2, 3, 4)) * w.view(1, 6)).to(torch.float64)
+        d_eb = ((g.double() - Cq.double()).sum(dim=(2, 3, 4)) * w.view(1, 6).double())
+        d_eb = d_eb.to(device=ctx.eb_device, dtype=ctx.eb_dtype)
         return d_lam.reshape(lmbda.shape), d_mu.reshape(mu.shape), d_eb, None, None, None, None
 
 

@@ -22,18 +22,16 @@ __all__ = ["lippmann_schwinger_cuda", "effective_tensor"]
 def lippmann_schwinger_cuda(
     lmbda, mu, epsilon_bar, grid_spec, rtol=1e-6, atol=1.0e-20, maxiter=32, verbose=0
 ):
-    """Wrapper for the HIP Lippmann Schwinger solver
+    """Solve for the strain with mean epsilon_bar on the GPU through the C ABI
+    of liblippmannschwinger.so (the in-tree build, else one on the loader path).
 
-    Requires liblippmannschwinger.so (built in-tree, or on LD_LIBRARY_PATH).
-
-    :arg lmbda: spatially varying Lame parameter lambda
-    :arg mu: spatially varying Lame parameter mu
-    :arg epsilon_bar: mean value of epsilon
-    :arg grid_spec: grid specification
-    :arg rtol: relative tolerance on normalised stress divergence to check convergence
-    :arg atol: absolute tolerance on normalised stress divergence to check convergence
-    :arg maxiter: maximal number of iterations
-    :arg verbose: verbosity level
+    lmbda, mu are float32 C-ordered fields of grid_spec's shape; epsilon_bar is
+    the Voigt mean strain (6 entries, tensor shear).  The loop stops once the
+    equilibrium residual, relative to the residual of the uniform start, falls
+    below rtol or below atol; it runs at most maxiter strain updates.  verbose 0
+    prints nothing, 1 a one-line summary, 2 the residual of every iteration.
+    Returns (epsilon, sigma, iterations); raises RuntimeError when the library
+    cannot be loaded, reports an error, or the iteration count reaches maxiter.
     """
     lib = _lib.load()
     nd = np.ctypeslib.ndpointer

@@ -92,3 +92,66 @@ def test_adjoint_matches_closed_form(gpu, n):
     em = np.linalg.norm(g_mu - cm) / np.linalg.norm(cm)
     print(f"{n}^3 adjoint vs closed form: lambda {el:.2e} mu {em:.2e}", flush=True)
     assert el < 1e-4 and em < 1e-4
+
+
+def _eps_bar_grad(n, seed, rtol=1e-8):
+    """d L / d eps_bar by the adjoint for L = sum(W * <sigma>(eps_bar)), and C_eff of the same grid."""
+    from jaxmaterials_amd.solver.adjoint import hooke_torch
+
+    gs = GridSpec(n, n, n, float(n), float(n), float(n))
+    lam, mu = crossply(n, seed)
+    solver = LSSolver(gs, nrhs=6)
+    rng = np.random.default_rng(n)
+    eb0 = rng.standard_normal((6, 6))
+    W = rng.standard_normal((6, 6))
+    # eps_bar stays on the host (as the reference's jacfwd input is), W on the GPU
+    eb = torch.tensor(eb0, dtype=torch.float64, requires_grad=True)
+    eps = solve_strain(lam, mu, eb, solver=solver, rtol=rtol, adjoint_rtol=rtol)
+    smean = hooke_torch(lam.unsqueeze(0), mu.unsqueeze(0), eps).to(torch.float64).mean(dim=(2, 3, 4))
+    L = (torch.tensor(W, device=smean.device) * smean).sum()
+    L.backward()
+    assert eb.grad is not None and eb.grad.device == eb.device and eb.grad.dtype == torch.float64
+    with torch.no_grad():
+        C = effective_tensor_torch(lam, mu, solver, rtol=rtol).cpu().numpy()
+    solver.close()
+    return lam, mu, gs, eb0, W, eb.grad.numpy(), C
+
+
+@pytest.mark.parametrize("n", [32, 128])
+def test_eps_bar_gradient_is_the_effective_tensor(gpu, n):
+    """The reference's only derivative, jax.jacfwd of <sigma> with respect to
+    eps_bar (src/driver.py:73-76, tests/test_yc.py:135-137), has the closed form
+    d<sigma_i>/d eps_bar_k = C_eff[k][i] (row k = <sigma> under e_k).  For
+    L = sum_r,i W[r,i] <sigma_i>(eps_bar_r) the adjoint's eps_bar gradient must
+    equal W C_eff^T, with C_eff from a forward solve on the same grid."""
+    _, _, _, _, W, g, C = _eps_bar_grad(n, seed=7)
+    expect = W @ C.T
+    err = np.abs(g - expect).max() / np.abs(expect).max()
+    print(f"{n}^3 eps_bar gradient vs W C_eff^T: max rel err {err:.2e}", flush=True)
+    assert err < 1e-4
+
+
+def test_eps_bar_gradient_matches_fp64_finite_differences(gpu):
+    """The same gradient against central differences of the FP64 oracle
+    (relative step 1e-3 on every component of two load cases)."""
+    n = 12
+    lam, mu, gs, eb0, W, g, _ = _eps_bar_grad(n, seed=8)
+    lam64 = lam.cpu().numpy().astype(np.float64)
+    mu64 = mu.cpu().numpy().astype(np.float64)
+    ogs = O.GridSpec(n, n, n, float(n), float(n), float(n))
+
+    def mean_sigma(e):
+        _, sig, _, _ = O.lippmann_schwinger_rfft(lam64, mu64, e, ogs, rtol=1e-12, depth=4, maxiter=5000)
+        return sig.mean(axis=(1, 2, 3))
+
+    errs = []
+    for r in (0, 3):
+        for k in range(6):
+            h = 1e-3 * max(abs(eb0[r, k]), 1e-3)
+            ep, em = eb0[r].copy(), eb0[r].copy()
+            ep[k] += h
+            em[k] -= h
+            fd = float(W[r] @ (mean_sigma(ep) - mean_sigma(em))) / (2 * h)
+            errs.append(abs(g[r, k] - fd) / max(abs(fd), 1e-12))
+            print(f"eps_bar[{r},{k}]: adjoint {g[r, k]:.6e} fd {fd:.6e}", flush=True)
+    assert max(errs) < 1e-4, max(errs)

@@ -95,3 +95,53 @@ def test_uneven_batch_matches_single_solves_and_oracle(gpu):
             e = _rel_max(res["ceff"][j], np.asarray(C))
             print(f"48^3 sample vs FP64 oracle: C_eff max rel err {e:.2e}")
             assert e < 1e-5
+
+
+def test_graph_solve_after_autotune_recaptures(gpu):
+    """A graph solve, the autotuner (which changes the tiles and the base
+    divergence layout) and a second graph solve on the same buffers: the second
+    must equal a launch-by-launch solve with the new tiles, not replay the
+    chunk captured with the old ones."""
+    import torch
+
+    from jaxmaterials_amd.common import GridSpec
+    from jaxmaterials_amd.data import crossply_lame
+    from jaxmaterials_amd.solver.device import LSSolver, unit_load_cases
+
+    n = 48
+    lam, mu, _, _ = crossply_lame((n,) * 3, (float(n),) * 3, 3, 3.0, 0.4, seed=21)
+    s = LSSolver(GridSpec(n, n, n, float(n), float(n), float(n)), nrhs=6, depth=4)
+    eps = torch.empty((6, 6, n, n, n), dtype=torch.float32, device=gpu)
+    first = s.solve(lam, mu, unit_load_cases(), epsilon=eps, rtol=1e-6, maxiter=2000, graph=True)
+    report = s.autotune(lam, mu)
+    print("autotune 48^3:", report)
+    again = s.solve(lam, mu, unit_load_cases(), epsilon=eps, rtol=1e-6, maxiter=2000, graph=True)
+    plain = s.solve(lam, mu, unit_load_cases(), rtol=1e-6, maxiter=2000, graph=False)
+    assert np.array_equal(again["iters"], plain["iters"])
+    assert np.array_equal(again["sigma_mean"], plain["sigma_mean"])
+    assert np.all(np.abs(again["iters"] - first["iters"]) <= 1)
+    assert _rel_max(again["sigma_mean"], first["sigma_mean"]) < 1e-5
+    s.close()
+
+
+def test_uneven_batch_large_grids_match_single_solves(gpu):
+    """One member each of the family's 64^3, 96^3 (radix 3) and 128^3 grids at
+    the configuration's Anderson depth 4, batched (graph replay, autotuned
+    tiles), against one launch-by-launch solve per microstructure."""
+    from jaxmaterials_amd.common import GridSpec
+    from jaxmaterials_amd.data.batch import lame_fields, solve_batch
+    from jaxmaterials_amd.solver.device import LSSolver, unit_load_cases
+
+    params = _family_samples((64, 96, 128), 1)
+    res = solve_batch(params, depth=4, graph=True, autotune=True, log=print)
+    for j, p in enumerate(params):
+        n = p["grid"]
+        lam, mu, _, _ = lame_fields(p, device=gpu)
+        s = LSSolver(GridSpec(n, n, n, float(n), float(n), float(n)), nrhs=6, depth=4)
+        one = s.solve(lam, mu, unit_load_cases(), rtol=1e-6, maxiter=3000)
+        s.close()
+        d = _rel_max(res["ceff"][j], one["sigma_mean"])
+        print(f"sample {p['index']} grid {n} ratio {p['mu_ratio']:.1f}: iters {res['iters'][j].tolist()} "
+              f"single {one['iters'].tolist()} C_eff rel diff {d:.2e}", flush=True)
+        assert np.all(np.abs(res["iters"][j] - one["iters"]) <= 1)
+        assert d < 1e-5
