@@ -197,6 +197,36 @@ int ls_strain_energy(const int *voxels, const float *d_lambda, const float *d_mu
                      const float *d_eps, double *d_psi, double *d_history,
                      void *stream);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The same solve as ls_solver_create(voxels, extents,
+ * nrhs, 1, depth, stream) + ls_solver_solve(..., rhs_mat NULL, d_tau NULL,
+ * warm_start 0, rel_ref 0, flags 0, verbose 0, ...) + ls_solver_destroy, with
+ * every internal buffer (spectrum, strain base, base divergence, Anderson
+ * rings, reduction partials, FFT tables) carved from one file-scope device
+ * array of LS_NOALLOC_SCRATCH_BYTES per GPU, which the library holds from its
+ * load until the process ends.  It calls no hipMalloc, hipFree, hipHostMalloc
+ * or other allocating HIP routine and creates no stream, event or graph; the
+ * host-side plan is rebuilt (ordinary heap memory) only when the grid,
+ * extents, nrhs or depth differ from the previous call on the device.
+ * Device pointers d_lambda, d_mu [N], d_eps [nrhs][6][N] (out), d_sigma
+ * [nrhs][6][N] or NULL; host eps_bar[nrhs][6]; iters, residual, sigma_mean as
+ * in ls_solver_solve (may be NULL).  stream = hipStream_t (NULL: the null
+ * stream); synchronous.  Returns 0 / 1 (not all converged) / -1 on error,
+ * including a problem whose buffers exceed the array (the message states the
+ * bytes needed).  Every call on a GPU uses that GPU's one array, which keeps
+ * the last call's contents (the solve initialises everything it reads).  Two
+ * calls that overlapped on one GPU would race on the array; the library
+ * serialises the calls of all threads of a process with a lock and returns
+ * only after the solve's work on `stream` has finished, so they cannot
+ * overlap within a process (each process has its own copy of the array). */
+#define LS_NOALLOC_SCRATCH_BYTES (128ull << 20)
+int ls_solve_noalloc(const int *voxels, const float *extents, int nrhs,
+                     int depth, const float *d_lambda, const float *d_mu,
+                     const float *eps_bar, float *d_eps, float *d_sigma,
+                     float rtol, float atol, int maxiter, int *iters,
+                     double *residual, double *sigma_mean, void *stream);
+
 /* Measured HBM streaming ceilings of the current device (a measurement aid,
  * no counterpart in the reference): grid-stride float4 kernels with
  * non-temporal loads and stores over `bytes`-sized arrays (three are

@@ -58,6 +58,11 @@ SIGNATURES = {
          ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p],
     ),
     "ls_hbm_stream_probe": (ctypes.c_int, [ctypes.c_longlong, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
+    "ls_solve_noalloc": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 5
+        + [ctypes.c_float, ctypes.c_float, ctypes.c_int] + [ctypes.c_void_p] * 4,
+    ),
     # phase-field fracture
     "ls_solver_set_damage": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float]),
     "ls_solver_phase_field": (

@@ -16,6 +16,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -266,15 +268,56 @@ struct ls_solver {
   bool timing = false;
   std::vector<hipEvent_t> tev;
   std::vector<int> tkind;
+  // built in a caller-owned device region (ls_solve_noalloc): no device
+  // buffer is freed, host buffers are heap memory, no stream or event is made
+  bool arena_mode = false;
+  std::vector<std::unique_ptr<unsigned char[]>> host_mem;
 };
 
 namespace {
+
+// Where the device buffers of a solver under construction come from: hipMalloc,
+// or (while t_arena is set: the no-allocation entry point) consecutive
+// 256-byte-aligned pieces of a caller-owned region.  An arena that runs out
+// hands out null pointers, keeps counting the bytes the solver needs and
+// flags the overflow, so the caller can report the size and give up.
+struct Arena {
+  unsigned char* base = nullptr;
+  size_t cap = 0, off = 0;
+  bool overflow = false;
+};
+thread_local Arena* t_arena = nullptr;
 
 template <typename T>
 int dalloc(T** p, size_t count) {
   *p = nullptr;
   if (count == 0) return 0;
+  if (t_arena) {
+    Arena& A = *t_arena;
+    const size_t off = (A.off + 255) & ~(size_t)255;
+    A.off = off + count * sizeof(T);
+    if (A.off > A.cap)
+      A.overflow = true;
+    else
+      *p = reinterpret_cast<T*>(A.base + off);
+    return 0;
+  }
   LS_HIP(hipMalloc((void**)p, count * sizeof(T)));
+  return 0;
+}
+
+
+// Host buffers the stream copies into: pinned memory, or ordinary heap memory
+// owned by the solver when it is built in an arena (no HIP allocation).
+template <typename T>
+int halloc(ls_solver* s, T** p, size_t count) {
+  *p = nullptr;
+  if (s->arena_mode) {
+    s->host_mem.emplace_back(new unsigned char[std::max<size_t>(1, count * sizeof(T))]);
+    *p = reinterpret_cast<T*>(s->host_mem.back().get());
+    return 0;
+  }
+  LS_HIP(hipHostMalloc((void**)p, count * sizeof(T), hipHostMallocDefault));
   return 0;
 }
 
@@ -304,6 +347,7 @@ int build_axis(HostAxis& a, int n, bool generic) {
   if (dalloc(&a.d_tw, n) || dalloc(&a.d_tw64, n) || dalloc(&a.d_s, n) || dalloc(&a.d_c, n) ||
       dalloc(&a.d_ph, n))
     return -1;
+  if (t_arena && t_arena->overflow) return 0;  // sizes counted, nothing to copy into
   LS_HIP(hipMemcpy(a.d_ph, ph.data(), n * sizeof(float2), hipMemcpyHostToDevice));
   LS_HIP(hipMemcpy(a.d_tw, tw.data(), n * sizeof(float2), hipMemcpyHostToDevice));
   LS_HIP(hipMemcpy(a.d_tw64, tw64.data(), n * sizeof(double2), hipMemcpyHostToDevice));
@@ -591,6 +635,12 @@ std::vector<int> active_mask(const ls_solver* s) {
 void free_solver(ls_solver* s) {
   if (s && s->gexec) (void)hipGraphExecDestroy(s->gexec);
   if (!s) return;
+  if (s->arena_mode) {
+    // device buffers belong to the caller's region, host buffers to host_mem
+    for (auto e : s->tev) (void)hipEventDestroy(e);
+    delete s;
+    return;
+  }
   for (auto& a : s->ax) {
     (void)hipFree(a.d_tw);
     (void)hipFree(a.d_tw64);
@@ -654,7 +704,7 @@ int create_impl(ls_solver* s, const int* voxels, const float* extents, int nrhs,
   s->depth = depth;
   s->m1 = depth + 1;
   if (const char* rb = std::getenv("LS_REBASE_FACTOR")) s->rebase_factor = std::atof(rb);
-  if (stream) {
+  if (stream || s->arena_mode) {
     s->stream = (hipStream_t)stream;
   } else {
     LS_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
@@ -750,15 +800,15 @@ int create_impl(ls_solver* s, const int* voxels, const float* extents, int nrhs,
   g.base = s->base;
   g.dmg = nullptr;  // Hooke's law until ls_solver_set_damage
   g.kdmg = 0.f;
-  LS_HIP(hipHostMalloc((void**)&s->h_st, nrhs * sizeof(RhsState), hipHostMallocDefault));
+  if (halloc(s, &s->h_st, nrhs)) return -1;
   s->chunk_stress = 8192;
   s->nblk_stress = (int)((s->N + s->chunk_stress - 1) / s->chunk_stress);
   if (dalloc(&s->part6, (size_t)nrhs * s->nblk_stress * 6) || dalloc(&s->d_mean6, (size_t)nrhs * 6)) return -1;
-  LS_HIP(hipHostMalloc((void**)&s->h_mean6, nrhs * 6 * sizeof(double), hipHostMallocDefault));
+  if (halloc(s, &s->h_mean6, (size_t)nrhs * 6)) return -1;
   s->chunk_mm = 65536;
   s->nblk_mm = (int)((s->N + s->chunk_mm - 1) / s->chunk_mm);
   if (dalloc(&s->d_mm, (size_t)nmat * s->nblk_mm * 4)) return -1;
-  LS_HIP(hipHostMalloc((void**)&s->h_mm, (size_t)nmat * s->nblk_mm * 4 * sizeof(float), hipHostMallocDefault));
+  if (halloc(s, &s->h_mm, (size_t)nmat * s->nblk_mm * 4)) return -1;
   if (depth > 0) {
     const size_t hist = (size_t)nrhs * s->m1 * 6 * s->N;
     // y ring in the d_base layout; Gram-row partials per k_aa_gram block
@@ -983,7 +1033,7 @@ int solve_impl(ls_solver* s, const float* d_lambda, const float* d_mu, const int
   }
   if (put_state(s)) return -1;
   if (reset_aa(s)) return -1;
-  s->timing = (std::getenv("LS_TIMING") != nullptr);
+  s->timing = !s->arena_mode && (std::getenv("LS_TIMING") != nullptr);
   mark(s, 3);
   const double drtol = rtol, datol = atol;
   std::vector<int> all(s->nrhs, 1);
@@ -1295,11 +1345,99 @@ int autotune_impl(ls_solver* s, const float* lam, const float* mu, float* eps, i
   return 0;
 }
 
+// ---- no-allocation solves -------------------------------------------------
+// Device memory of ls_solve_noalloc: one array per GPU, held from the
+// library's load until the process ends.  Host code never passes `scratch`
+// itself (in host code the name is a stand-in, not a device address); it uses
+// the address hipGetSymbolAddress returns on each device.
+constexpr size_t kScratchBytes = LS_NOALLOC_SCRATCH_BYTES;
+constexpr int kScratchDevices = 64;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// Per device: the array's address and the solver last built in it (kept while
+// the grid, extents, load-case count and depth stay the same, so repeated
+// calls allocate nothing, not even host memory).
+struct ScratchSlot {
+  unsigned char* ptr = nullptr;
+  ls_solver* s = nullptr;
+  int vox[3] = {0, 0, 0};
+  float ext[3] = {0.f, 0.f, 0.f};
+  int nrhs = 0, depth = -1;
+};
+ScratchSlot g_scratch[kScratchDevices];
+std::mutex g_scratch_mu;  // serialises the calls of all threads of the process
+
+int solve_noalloc_impl(const int* voxels, const float* extents, int nrhs, int depth, const float* d_lambda,
+                       const float* d_mu, const float* eps_bar, float* d_eps, float* d_sigma, float rtol,
+                       float atol, int maxiter, int* iters, double* residual, double* sigma_mean, void* stream) {
+  if (!voxels || !extents || !eps_bar || !d_lambda || !d_mu || !d_eps) {
+    fprintf(stderr, "liblippmannschwinger: ls_solve_noalloc: null argument\n");
+    return -1;
+  }
+  int dev = 0;
+  LS_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kScratchDevices) {
+    fprintf(stderr, "liblippmannschwinger: ls_solve_noalloc: device %d beyond %d\n", dev, kScratchDevices);
+    return -1;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  ScratchSlot& slot = g_scratch[dev];
+  if (!slot.ptr) {
+    // once per device, with that device current
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      fprintf(stderr, "liblippmannschwinger: ls_solve_noalloc: hipGetSymbolAddress(scratch) failed: %s\n",
+              hipGetErrorName(e));
+      return -1;
+    }
+    slot.ptr = static_cast<unsigned char*>(p);
+  }
+  const bool same = slot.s && slot.nrhs == nrhs && slot.depth == depth &&
+                    std::equal(voxels, voxels + 3, slot.vox) && std::equal(extents, extents + 3, slot.ext);
+  if (!same) {
+    free_solver(slot.s);  // host side only: its device buffers lie in the array
+    slot.s = nullptr;
+    ls_solver* s = new ls_solver();
+    s->arena_mode = true;
+    Arena arena;
+    arena.base = slot.ptr;
+    arena.cap = kScratchBytes;
+    t_arena = &arena;
+    const int rc = create_impl(s, voxels, extents, nrhs, 1, depth, stream);
+    t_arena = nullptr;
+    if (rc || arena.overflow) {
+      if (!rc)
+        fprintf(stderr,
+                "liblippmannschwinger: ls_solve_noalloc: grid %d x %d x %d with %d load cases and depth %d needs "
+                "%zu bytes of scratch, the library holds %zu\n",
+                voxels[0], voxels[1], voxels[2], nrhs, depth, arena.off, kScratchBytes);
+      free_solver(s);
+      return -1;
+    }
+    slot.s = s;
+    std::copy(voxels, voxels + 3, slot.vox);
+    std::copy(extents, extents + 3, slot.ext);
+    slot.nrhs = nrhs;
+    slot.depth = depth;
+  }
+  slot.s->stream = (hipStream_t)stream;
+  return solve_impl(slot.s, d_lambda, d_mu, nullptr, eps_bar, nullptr, d_eps, d_sigma, 0, rtol, atol, maxiter, 0.0, 0,
+                    iters, residual, sigma_mean, false, 0);
+}
+
 }  // namespace
 
 // ============================================================================
 // C ABI
 extern "C" {
+
+int ls_solve_noalloc(const int* voxels, const float* extents, int nrhs, int depth, const float* d_lambda,
+                     const float* d_mu, const float* eps_bar, float* d_eps, float* d_sigma, float rtol, float atol,
+                     int maxiter, int* iters, double* residual, double* sigma_mean, void* stream) {
+  return solve_noalloc_impl(voxels, extents, nrhs, depth, d_lambda, d_mu, eps_bar, d_eps, d_sigma, rtol, atol,
+                            maxiter, iters, residual, sigma_mean, stream);
+}
 
 ls_solver* ls_solver_create(const int* voxels, const float* extents, int nrhs, int nmat, int depth, void* stream) {
   if (!voxels || !extents) return nullptr;
