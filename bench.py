@@ -25,7 +25,7 @@ sys.path.insert(0, ROOT)
 METRIC = "C_eff solves/sec to 1e-6 residual at 256³ (plain vs batched AA + learned start)"
 
 
-def model_bytes(kind, N, M, iters, depth, nrebase=0):
+def model_bytes(kind, N, M, iters, depth, nrebase=0, cg=False):
     """Algorithmic HBM bytes of all launches of one kernel kind over a solve.
 
     One HBM round trip of the z-half spectrum (48 M bytes per load case) per
@@ -43,18 +43,27 @@ def model_bytes(kind, N, M, iters, depth, nrebase=0):
           Anderson also the residual's three-complex-per-mode form y written 24 M
       aa_gram : Anderson Gram row, the f newest y read, f * 24 M,
           f = min(e + 1, depth + 1) at the e-th evaluation
+    Conjugate gradients (cg):
+      z_cg : spectrum 96 M + correction read and written 48 N + direction p
+             read and written 48 N (+ Lame 8 N shared)
+      x    : 144 M per evaluation: the spectrum read and written and the
+             running divergence read and written
     """
     its = [int(i) for i in iters]
     m1 = depth + 1
     nb = max(its) if its else 0
     if kind == "z":
-        return sum(k * (96 * M + 48 * N) for k in its) + nb * 8 * N if depth == 0 else 0
+        return sum(k * (96 * M + 48 * N) for k in its) + nb * 8 * N if depth == 0 and not cg else 0
+    if kind == "z_cg":
+        return sum(k * (96 * M + 96 * N) for k in its) + nb * 8 * N if cg else 0
     if kind == "z_aa":
         return sum(96 * M + 24 * N + 24 * N * (min(j, m1) - 1) + (24 * N if j > m1 else 0)
                    for k in its for j in range(1, k + 1)) + nb * 8 * N
     if kind == "y":
         return sum(2 * k for k in its) * 96 * M
     if kind == "x":
+        if cg:
+            return (sum(k + 1 for k in its) + nrebase * len(its)) * 144 * M
         # spectrum in and out (96 M) and the base divergence d_base read (24 M)
         b = (sum(k + 1 for k in its) + nrebase * len(its)) * 120 * M
         if depth > 0:
@@ -65,8 +74,8 @@ def model_bytes(kind, N, M, iters, depth, nrebase=0):
     return 0
 
 
-KERNEL_OF = {"z": "k_zline<Z_UPD>", "z_aa": "k_zline<Z_AA>", "y": "k_ypass_r", "x": "k_xpass_h",
-             "aa_gram": "k_aa_gram"}
+KERNEL_OF = {"z": "k_zline<Z_UPD>", "z_aa": "k_zline<Z_AA>", "z_cg": "k_zline<Z_CG>", "y": "k_ypass_r",
+             "x": "k_xpass_h", "aa_gram": "k_aa_gram"}
 
 
 def measured_traffic(kind, n, method):
@@ -360,9 +369,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--grid", type=int, default=256)
     ap.add_argument("--depth", type=int, default=6)
-    ap.add_argument("--methods", default="plain,aa,aa5,aa_ws",
+    ap.add_argument("--methods", default="plain,aa,aa5,aa_ws,cg,cg_ws",
                     help="plain: no Anderson; aa: Anderson of --depth; aaK: Anderson of depth K (aa5 is the "
-                         "configuration's m = 5); a suffix _ws starts from the learned warm start")
+                         "configuration's m = 5); cg: conjugate gradients; a suffix _ws starts from the learned "
+                         "warm start")
     ap.add_argument("--rtol", type=float, default=1e-6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", choices=("crossply256", "dataset", "phasefield"), default="crossply256",
@@ -412,7 +422,8 @@ def main():
     for method in args.methods.split(","):
         warm = method.endswith("_ws")
         base_name = method[:-3] if warm else method
-        if base_name == "plain":
+        use_cg = base_name == "cg"
+        if base_name in ("plain", "cg"):
             depth = 0
         elif base_name == "aa":
             depth = args.depth
@@ -430,7 +441,7 @@ def main():
             if net is None:
                 net = WarmStartNet.load(DEFAULT_WEIGHTS)
             r_init = warm_start_solve(net, solver, lam, mu, loads, epsilon=eps, rtol=args.rtol, maxiter=0,
-                                      check=False)["residual"]
+                                      check=False, cg=use_cg)["residual"]
             eb6 = torch.eye(6, device="cuda")
             torch.cuda.synchronize()
             t0 = time.perf_counter()
@@ -442,8 +453,9 @@ def main():
 
         def run():
             if warm:
-                return warm_start_solve(net, solver, lam, mu, loads, epsilon=eps, rtol=args.rtol, maxiter=3000)
-            return solver.solve(lam, mu, loads, epsilon=eps, rtol=args.rtol, maxiter=3000)
+                return warm_start_solve(net, solver, lam, mu, loads, epsilon=eps, rtol=args.rtol, maxiter=3000,
+                                        cg=use_cg)
+            return solver.solve(lam, mu, loads, epsilon=eps, rtol=args.rtol, maxiter=3000, cg=use_cg)
 
         for _ in range(args.warmup):
             run()
@@ -467,17 +479,18 @@ def main():
         C = out["sigma_mean"]
         sec = float(np.mean(times))
         kinds = {}
-        for name in ("z", "z_aa", "y", "x", "aa_gram"):
+        for name in ("z", "z_aa", "z_cg", "y", "x", "aa_gram"):
             if cnt_kind[name] == 0:
                 continue
-            b = model_bytes(name, N, M, iters, depth, nreb)
+            b = model_bytes(name, N, M, iters, depth, nreb, use_cg)
             t = ms_kind[name] / 1e3 / args.steps
             kinds[name] = {"kernel": KERNEL_OF[name], "ms_total": t * 1e3,
                            "launches": int(cnt_kind[name] // args.steps),
                            "avg_launch_ms": t * 1e3 / max(1, cnt_kind[name] // args.steps),
                            "algorithmic_bytes": b, "achieved_GBps": (b / t / 1e9) if t > 0 else None}
-        loop_bytes = sum(model_bytes(k, N, M, iters, depth, nreb) for k in kinds)
+        loop_bytes = sum(model_bytes(k, N, M, iters, depth, nreb, use_cg) for k in kinds)
         rec = {"solves_per_s": 1.0 / sec, "ms_per_solve": sec * 1e3, "iters": iters, "depth": depth,
+               "accelerator": "cg" if use_cg else ("anderson" if depth > 0 else "none"),
                "base_reevaluations": nreb, "passes": kinds, "loop_GBps": loop_bytes / sec / 1e9,
                "ceff": C.tolist()}
         if ws_info is not None:

@@ -70,10 +70,15 @@ ls_solver *ls_solver_create(const int *voxels, const float *extents, int nrhs,
  * adjoint, whose stress may have zero mean); LS_GRAPH = run the iterations
  * between two host synchronisations as one replay of a captured HIP graph
  * (captured on first use, again when the fields or tolerances change; the
- * results are identical to the launch-by-launch path).  Returns 0 if all
- * converged, 1 if some did not, -1 on error. */
+ * results are identical to the launch-by-launch path); LS_CG = conjugate
+ * gradients instead of the Lippmann-Schwinger / Anderson update (the same
+ * equation, residual, stop test and iteration count semantics: Gamma0 C is
+ * self-adjoint and positive definite in the reference medium's inner
+ * product on compatible fields; needs Hooke's law, any depth).  Returns 0 if
+ * all converged, 1 if some did not, -1 on error. */
 #define LS_ABSOLUTE_RESIDUAL 1
 #define LS_GRAPH 2
+#define LS_CG 4
 int ls_solver_solve(ls_solver *s, const float *d_lambda, const float *d_mu,
                     const int *rhs_mat, const float *eps_bar,
                     const float *d_tau, float *d_eps, float *d_sigma,
@@ -95,7 +100,8 @@ int ls_solver_autotune(ls_solver *s, const float *d_lambda, const float *d_mu,
  * stream when the environment variable LS_TIMING is set), ms and launch
  * counts per pass kind: [0] z-line passes (plain), [1] y passes, [2] x passes,
  * [3] everything else, [4] z-line Anderson passes (update, mixing, Hooke,
- * forward transform), [5] Anderson Gram-row reductions.  Fills the first
+ * forward transform), [5] Anderson Gram-row reductions, [6] z-line
+ * conjugate-gradient passes.  Fills the first
  * nkinds kinds; returns the number of base re-evaluations of the last solve. */
 int ls_solver_last_timing(ls_solver *s, double *ms, int *count, int nkinds);
 

@@ -115,6 +115,38 @@ enum ZMode : int {
   Z_SET = 3,    // C2R -> eps = eps_bar - r (end of a projection; no forward transform)
   Z_PROJ = 4,   // eps read, sigma = C0 (eps - eps_bar), R2C  (start of a projection)
   Z_AA = 5,     // C2R -> r; g = delta + r into the newest slot; next delta = sum_s alpha_s g^(s) -> C delta -> R2C
+  Z_CG = 6,     // C2R -> r; delta += alpha p; p = r + beta p -> C p -> R2C, block partials of <p, C p>_w
+};
+
+// Conjugate gradients on the same equation (LS_CG).  On compatible fields
+// with zero mean, A = Gamma0 C is self-adjoint and positive definite in the
+// reference-medium product <a, b>_C0 = <a, C0 b>_w (Gamma0 C0 is the
+// C0-orthogonal projection onto them), and the Lippmann-Schwinger update
+// r = -Gamma0 (C eps) is the residual b - A e of eps = eps_bar + e.  Per load
+// case and iteration k:
+//   z pass     r_k from the inverse transform; delta += alpha_{k-1} p_{k-1}
+//              (the update of the previous step, applied one pass late);
+//              p_k = r_k + beta_{k-1} p_{k-1}; sigma = C p_k, R2C; and
+//              <p_k, A p_k>_C0 = <p_k, C p_k>_w (pointwise, Gamma0 C0 p = p);
+//   k_cg_alpha alpha_k = rr_k / <p_k, C p_k>_w;
+//   x pass     d += alpha_k xi~ . (C p_k)^ (d, the divergence of the
+//              iterate's stress, kept in the base-divergence buffer: at a
+//              (re)start it is the FP64 base divergence); the residual R from
+//              d; rr_{k+1} = <r, C0 r>_w from d per mode,
+//              sum_k w_k (|d|^2 - c |n . d|^2) / (mu0 |xi~|^2) / N,
+//              c = (lambda0 + mu0) / (lambda0 + 2 mu0) (r_hat = -Gamma0
+//              sigma_hat is a function of d; the self-conjugate planes need no
+//              correction: r_hat of a real stress is Hermitian there);
+//              r_hat_{k+1} = -Gamma0 from d;
+//   k_cg_beta  beta_k = rr_{k+1} / rr_k (0 at a restart).
+// A rebase (the FP64 re-evaluation of the correction formulation) restarts
+// the recurrence from the exact divergence of the new base.
+struct CgState {
+  double alpha;  // the pending update delta += alpha p (not yet applied when applied == 0)
+  double beta;
+  double rr;     // <r, C0 r>_w of the current residual
+  int restart;   // next beta is 0
+  int applied;   // the pending update has been applied (by a z pass or k_cg_finish)
 };
 
 // Anderson history (lippmann_schwinger.py:177-231, doc/linear_elasticity.tex:335-370).
@@ -148,6 +180,11 @@ struct AAParams {
   const int* filled;    // [R] number of filled slots including the newest
   int m1;               // depth + 1
   int record;           // x pass: store y of the newest residual
+  // conjugate gradients: direction p [R][6][N], state per load case, block
+  // partials (z pass: <p, C p>_w; x pass: the C0 norm of the residual)
+  float* P;
+  CgState* cg;
+  double* cg_part;
 };
 
 // The self-conjugate planes k_z = 0 and (nz even) k_z = nz/2.  There the
@@ -325,7 +362,13 @@ __global__ void __launch_bounds__(NT) k_zline(GridDev g, const RhsDesc* __restri
   const int tid = threadIdx.x;
   const FastDiv f_none = g.f_nz;  // unused by the line-contiguous layout
 
-  if constexpr (MODE == Z_UPD || MODE == Z_SET || MODE == Z_AA) {
+  [[maybe_unused]] float cga = 0.f, cgb = 0.f;
+  [[maybe_unused]] double pcp = 0.0;
+  if constexpr (MODE == Z_CG) {
+    cga = (float)aa.cg[r].alpha;
+    cgb = (float)aa.cg[r].beta;
+  }
+  if constexpr (MODE == Z_UPD || MODE == Z_SET || MODE == Z_AA || MODE == Z_CG) {
     // gather the half spectra into the paired full-length lines (C2R semantics:
     // imaginary parts of the k = 0 and k = nz/2 coefficients are ignored)
     const int items = L * P * 3;
@@ -410,6 +453,23 @@ __global__ void __launch_bounds__(NT) k_zline(GridDev g, const RhsDesc* __restri
           stnt(&Gh[c * N + v], gn[c]);
           ev[c] = fmaf(a0, gn[c], dn[c]);
         }
+      } else if constexpr (MODE == Z_CG) {
+        // r from the inverse transform; delta += alpha p (pending update);
+        // p = r + beta p, whose stress is transformed
+        float* Pr = aa.P + (long long)r * 6 * N;
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+          const float2 z = lds[lds_idx<true>(l * 3 + p, j, 1, ls)];
+          ev[2 * p] = z.x;
+          ev[2 * p + 1] = z.y;
+        }
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+          const float po = ldnt(&Pr[c * N + v]);
+          stnt(&E_[c * N + v], fmaf(cga, po, ldnt(&E_[c * N + v])));
+          ev[c] = fmaf(cgb, po, ev[c]);
+          stnt(&Pr[c * N + v], ev[c]);
+        }
       } else {
         float rr[6];
 #pragma unroll
@@ -454,10 +514,21 @@ __global__ void __launch_bounds__(NT) k_zline(GridDev g, const RhsDesc* __restri
 #pragma unroll
             for (int c = 0; c < 6; ++c) s[c] += (float)(gm1 * (sp1[c] - sp0[c]));
           }
+          if constexpr (MODE == Z_CG) {
+            pcp += (double)ev[0] * s[0] + (double)ev[1] * s[1] + (double)ev[2] * s[2] +
+                   2.0 * ((double)ev[3] * s[3] + (double)ev[4] * s[4] + (double)ev[5] * s[5]);
+          }
         }
 #pragma unroll
         for (int p = 0; p < 3; ++p) lds[lds_idx<true>(l * 3 + p, j, 1, ls)] = make_float2(s[2 * p], s[2 * p + 1]);
       }
+    }
+  }
+  if constexpr (MODE == Z_CG) {
+    const double bs = block_sum<NT>(pcp, red);
+    if (tid == 0) {
+      aa.cg_part[(long long)r * ntiles + tile] = bs;
+      if (tile == 0) aa.cg[r].applied = 1;
     }
   }
   if constexpr (MODE == Z_SET) {
@@ -533,15 +604,18 @@ __device__ __forceinline__ void xi_tilde(const GridDev& g, int kx, int ky, int k
 // the divergence: n = xi~/|xi~|, v = d/|xi~| (= sigma_hat n), rho = n . v,
 //   r_hat = cA (n0 v0, n1 v1, n2 v2, (n0 v1 + n1 v0)/2, (n0 v2 + n2 v0)/2,
 //               (n1 v2 + n2 v1)/2) + cB rho (n0^2, n1^2, n2^2, n0 n1, n0 n2, n1 n2)
-template <int E, int NT, bool REC>
+// REC: 0 plain; 1 Anderson (record y of the residual); 2 conjugate gradients
+// (d = alpha xi~ . sigma_hat + d, written back, and the C0 norm of the residual).
+template <int E, int NT, int REC>
 __global__ void __launch_bounds__(NT) k_xpass(GridDev g, const RhsDesc* __restrict__ rhs,
                                               const RhsState* __restrict__ st, float2* __restrict__ spec,
-                                              const float2* __restrict__ dbase, double* __restrict__ partials,
+                                              float2* __restrict__ dbase, double* __restrict__ partials,
                                               float* __restrict__ dc, FastDiv f_Tm, FastDiv f_cols, int ntiles,
                                               DbLayout dl, int do_gamma, AAParams aa) {
   extern __shared__ float2 lds[];
   __shared__ double red[NT / 64];
-  const bool rec = REC && do_gamma;
+  const bool rec = REC == 1 && do_gamma;
+  constexpr bool CG = REC == 2;
   const int Tm = (int)f_Tm.d;
   const int lin = xcd_remap(blockIdx.x, gridDim.x);
   const int r = lin / ntiles;
@@ -552,10 +626,13 @@ __global__ void __launch_bounds__(NT) k_xpass(GridDev g, const RhsDesc* __restri
   const int m0 = tile * Tm;
   const int cols = 6 * Tm;
   float2* S = spec + (long long)r * g.M * 6;
-  const float2* DB = dbase ? dbase + (long long)r * dl.per_rhs + (long long)tile * nx * Tm * 3 : nullptr;
+  float2* DB = dbase ? dbase + (long long)r * dl.per_rhs + (long long)tile * nx * Tm * 3 : nullptr;
   const RhsDesc d = rhs[r];
   const int tid = threadIdx.x;
   const long long colmax = nm * 6;
+  [[maybe_unused]] const float cga = CG ? (float)aa.cg[r].alpha : 1.f;
+  [[maybe_unused]] const float ccg = CG ? (d.lam0 + d.mu0) / (d.lam0 + 2.f * d.mu0) : 0.f;
+  [[maybe_unused]] double acc2 = 0.0;
   const int items = nx * cols;
   for (int e = tid; e < items; e += NT) {
     const int x = fdiv(e, f_cols), t = e - x * cols;
@@ -590,8 +667,13 @@ __global__ void __launch_bounds__(NT) k_xpass(GridDev g, const RhsDesc* __restri
     d1.y = k0 * s[3].y + k1 * s[1].y + k2 * s[5].y;
     d2.x = k0 * s[4].x + k1 * s[5].x + k2 * s[2].x;
     d2.y = k0 * s[4].y + k1 * s[5].y + k2 * s[2].y;
+    if constexpr (CG) {
+      d0 = cscale(d0, cga);
+      d1 = cscale(d1, cga);
+      d2 = cscale(d2, cga);
+    }
     if (DB) {
-      const float2* b = DB + (x * Tm + m) * 3;
+      float2* b = DB + (x * Tm + m) * 3;
       const float2 b0 = b[0], b1 = b[1], b2 = b[2];
       d0.x += b0.x;
       d0.y += b0.y;
@@ -599,10 +681,16 @@ __global__ void __launch_bounds__(NT) k_xpass(GridDev g, const RhsDesc* __restri
       d1.y += b1.y;
       d2.x += b2.x;
       d2.y += b2.y;
+      if constexpr (CG) {
+        b[0] = d0;
+        b[1] = d1;
+        b[2] = d2;
+      }
     }
     const double wk = (kz == 0 || 2 * kz == g.nz) ? 1.0 : 2.0;
-    acc += wk * ((double)d0.x * d0.x + (double)d0.y * d0.y + (double)d1.x * d1.x + (double)d1.y * d1.y +
-                 (double)d2.x * d2.x + (double)d2.y * d2.y);
+    const double e2m = (double)d0.x * d0.x + (double)d0.y * d0.y + (double)d1.x * d1.x + (double)d1.y * d1.y +
+                       (double)d2.x * d2.x + (double)d2.y * d2.y;
+    acc += wk * e2m;
     if (do_gamma) {
       const float q2 = k0 * k0 + k1 * k1 + k2 * k2;
       float2 o[6];
@@ -611,6 +699,8 @@ __global__ void __launch_bounds__(NT) k_xpass(GridDev g, const RhsDesc* __restri
         const float n0 = k0 * inv, n1 = k1 * inv, n2 = k2 * inv;
         const float2 v0 = cscale(d0, inv), v1 = cscale(d1, inv), v2 = cscale(d2, inv);
         const float2 rho = make_float2(n0 * v0.x + n1 * v1.x + n2 * v2.x, n0 * v0.y + n1 * v1.y + n2 * v2.y);
+        if constexpr (CG)
+          acc2 += wk * (e2m / (double)q2 - (double)ccg * ((double)rho.x * rho.x + (double)rho.y * rho.y));
         const float cA = d.cA, cB = d.cB;
         const float h = 0.5f * cA;
         o[0] = make_float2(cA * n0 * v0.x + cB * rho.x * n0 * n0, cA * n0 * v0.y + cB * rho.y * n0 * n0);
@@ -646,6 +736,10 @@ __global__ void __launch_bounds__(NT) k_xpass(GridDev g, const RhsDesc* __restri
   }
   const double bs = block_sum<NT>(acc, red);
   if (tid == 0) partials[(long long)r * ntiles + tile] = bs;
+  if constexpr (CG) {
+    const double b2 = block_sum<NT>(acc2, red);
+    if (tid == 0) aa.cg_part[(long long)r * ntiles + tile] = b2;
+  }
   __syncthreads();
   if (!do_gamma) return;
   fft_lines<true, true, false, E, NT>(lds, g.px, f_cols, cols, cols, 1);
@@ -854,6 +948,82 @@ __global__ void __launch_bounds__(kThreads) k_check(RhsState* __restrict__ st, c
     if (mode != 2) q.checks += 1;
     st[r] = q;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Conjugate-gradient scalars (see CgState).  One block per load case.
+// alpha = rr / <p, C p>_w from the z pass partials; a non-positive or
+// non-finite <p, C p> (an indefinite medium or a NaN) stops the load case.
+__global__ void __launch_bounds__(kThreads) k_cg_alpha(RhsState* __restrict__ st, CgState* __restrict__ cg,
+                                                       const double* __restrict__ part, int nparts) {
+  __shared__ double red[kThreads / 64];
+  const int r = blockIdx.x;
+  if (!st[r].active) return;
+  const double* p = part + (long long)r * nparts;
+  double a = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += kThreads) a += p[i];
+  const double pcp = block_sum(a, red);
+  if (threadIdx.x == 0) {
+    CgState c = cg[r];
+    if (pcp > 0.0 && pcp < 1e300) {
+      c.alpha = c.rr / pcp;
+    } else {
+      c.alpha = 0.0;
+      if (pcp != 0.0) {  // a zero direction (exact solution) is no error
+        st[r].status = 1;
+        st[r].active = 0;
+      }
+    }
+    c.applied = 0;
+    cg[r] = c;
+  }
+}
+
+// beta = rr_new / rr from the x pass partials (rr_new = sum / (N mu0)); 0 at a restart.
+__global__ void __launch_bounds__(kThreads) k_cg_beta(const RhsState* __restrict__ st,
+                                                      const RhsDesc* __restrict__ rhs, CgState* __restrict__ cg,
+                                                      const double* __restrict__ part, int nparts, double invN) {
+  __shared__ double red[kThreads / 64];
+  const int r = blockIdx.x;
+  if (!st[r].active) return;
+  const double* p = part + (long long)r * nparts;
+  double a = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += kThreads) a += p[i];
+  const double s = block_sum(a, red);
+  if (threadIdx.x == 0) {
+    CgState c = cg[r];
+    const double rr = s * invN / (double)rhs[r].mu0;
+    c.beta = (c.restart || !(c.rr > 0.0)) ? 0.0 : rr / c.rr;
+    c.restart = 0;
+    c.rr = rr;
+    cg[r] = c;
+  }
+}
+
+// The pending update delta += alpha p of every load case whose last step was
+// an x pass (active or not), with the z pass's FP32 operation.
+__global__ void __launch_bounds__(kThreads) k_cg_finish(const CgState* __restrict__ cg, long long N,
+                                                        const float* __restrict__ P, float* __restrict__ delta) {
+  const int r = blockIdx.y;
+  const CgState c = cg[r];
+  if (c.applied || c.alpha == 0.0) return;
+  const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= 6 * N) return;
+  const long long off = (long long)r * 6 * N + i;
+  delta[off] = fmaf((float)c.alpha, P[off], delta[off]);
+}
+
+// (Re)start of the recurrence for the selected load cases (sel null: all):
+// no pending update, the next beta 0.
+__global__ void k_cg_reset(CgState* __restrict__ cg, const int* __restrict__ sel, int nrhs) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrhs || (sel && !sel[r])) return;
+  CgState c = cg[r];
+  c.alpha = 0.0;
+  c.beta = 0.0;
+  c.restart = 1;
+  c.applied = 1;
+  cg[r] = c;
 }
 
 // ---------------------------------------------------------------------------

@@ -131,10 +131,13 @@ namespace lsk {
 // INPL: the transforms overwrite the h / g lines in place (each thread holds
 // at most one line segment of stages 1 and 3 in registers across a barrier),
 // which halves the LDS of a block.
-template <int A, int B, int TM, int NT, bool REC, int MODE>
+// REC: 0 plain; 1 Anderson (record y of the residual); 2 conjugate gradients
+// (d = alpha e^{-i pi k/n} F[h] + d, written back to the base-divergence
+// buffer, and the block partial of the residual's C0 norm, see CgState).
+template <int A, int B, int TM, int NT, int REC, int MODE>
 __global__ void __launch_bounds__(NT) k_xpass_h(GridDev g, const RhsDesc* __restrict__ rhs,
                                                 const RhsState* __restrict__ st, float2* __restrict__ spec,
-                                                const float2* __restrict__ dbase, double* __restrict__ partials,
+                                                float2* __restrict__ dbase, double* __restrict__ partials,
                                                 float* __restrict__ dc, int ntiles, long long db_per_rhs,
                                                 int do_gamma, AAParams aa) {
   constexpr int N = A * B;
@@ -162,7 +165,8 @@ __global__ void __launch_bounds__(NT) k_xpass_h(GridDev g, const RhsDesc* __rest
   const int m0 = tile * TM;
   const long long colmax = 6LL * nm;
   float2* S = spec + (long long)r * g.M * 6 + 6LL * m0;
-  const float2* DB = dbase ? dbase + (long long)r * db_per_rhs + (long long)tile * N * TM * 3 : nullptr;
+  float2* DB = dbase ? dbase + (long long)r * db_per_rhs + (long long)tile * N * TM * 3 : nullptr;
+  constexpr bool CG = REC == 2;
   // the x twiddles e^{-2 pi i j / N} in LDS (indices t k1 < N of stages 1 and 2)
   __shared__ float2 tw[N];
   for (int i = threadIdx.x; i < N; i += NT) tw[i] = g.px.tw[i];
@@ -259,7 +263,10 @@ __global__ void __launch_bounds__(NT) k_xpass_h(GridDev g, const RhsDesc* __rest
   // stage 2: item (k1, m): B-point DFTs of h, residual and Gamma0 per kx,
   // inverse B-point DFTs of u' (with Anderson, y of u' is recorded for the Gram row)
   double acc = 0.0;
-  const bool rec = REC && do_gamma;
+  const bool rec = REC == 1 && do_gamma;
+  [[maybe_unused]] double acc2 = 0.0;
+  [[maybe_unused]] const float cga = CG ? (float)aa.cg[r].alpha : 1.f;
+  [[maybe_unused]] const float ccg = CG ? (d.lam0 + d.mu0) / (d.lam0 + 2.f * d.mu0) : 0.f;
   float2* Un = rec ? aa.U + ((long long)r * aa.m1 + aa.head[r]) * aa.u_per_rhs + (long long)tile * N * TM * 3
                    : nullptr;
   if constexpr (!SPLIT) {
@@ -281,14 +288,25 @@ __global__ void __launch_bounds__(NT) k_xpass_h(GridDev g, const RhsDesc* __rest
         float2 d0 = u[0][k2], d1 = u[1][k2], d2 = u[2][k2];
         // tile entry (kx, m, i) at ((k2 3 + i) TM + m) A + k1 (DbLayout, three-line order)
         const int eo = (k2 * 3 * TM + m) * A + k1;
+        if constexpr (CG) {
+          d0 = cscale(d0, cga);
+          d1 = cscale(d1, cga);
+          d2 = cscale(d2, cga);
+        }
         if (DB) {
           const float2 ph = g.phx[kx];
           d0 = cadd(d0, cmul(ph, DB[eo]));
           d1 = cadd(d1, cmul(ph, DB[eo + TM * A]));
           d2 = cadd(d2, cmul(ph, DB[eo + 2 * TM * A]));
+          if constexpr (CG) {
+            DB[eo] = cmulc(d0, ph);
+            DB[eo + TM * A] = cmulc(d1, ph);
+            DB[eo + 2 * TM * A] = cmulc(d2, ph);
+          }
         }
-        e2 += (double)d0.x * d0.x + (double)d0.y * d0.y + (double)d1.x * d1.x + (double)d1.y * d1.y +
-              (double)d2.x * d2.x + (double)d2.y * d2.y;
+        const double e2m = (double)d0.x * d0.x + (double)d0.y * d0.y + (double)d1.x * d1.x + (double)d1.y * d1.y +
+                           (double)d2.x * d2.x + (double)d2.y * d2.y;
+        e2 += e2m;
         const float sxk = g.sx[kx], cxk = g.cx[kx];
         const float k0 = al * sxk;
         const float kk1 = be * cxk;
@@ -298,6 +316,9 @@ __global__ void __launch_bounds__(NT) k_xpass_h(GridDev g, const RhsDesc* __rest
           const float inv = 1.0f / sqrtf(q2);
           const float n0 = k0 * inv, n1 = kk1 * inv, n2 = kk2 * inv;
           const float rhox = (n0 * d0.x + n1 * d1.x + n2 * d2.x) * inv, rhoy = (n0 * d0.y + n1 * d1.y + n2 * d2.y) * inv;
+          if constexpr (CG)
+            acc2 += (double)coef[m][3] *
+                    (e2m / (double)q2 - (double)ccg * ((double)rhox * rhox + (double)rhoy * rhoy));
           const float cA = d.cA * inv * inv, cB = d.cB * inv;
           u[0][k2] = make_float2(cA * d0.x + cB * rhox * n0, cA * d0.y + cB * rhoy * n0);
           u[1][k2] = make_float2(cA * d1.x + cB * rhox * n1, cA * d1.y + cB * rhoy * n1);
@@ -350,14 +371,25 @@ __global__ void __launch_bounds__(NT) k_xpass_h(GridDev g, const RhsDesc* __rest
                 p2 = L::at(3 * m + 2, k2 + B * k1);
       float2 d0 = W[p0], d1 = W[p1], d2 = W[p2];
       const int eo = (k2 * 3 * TM + m) * A + k1;
+      if constexpr (CG) {
+        d0 = cscale(d0, cga);
+        d1 = cscale(d1, cga);
+        d2 = cscale(d2, cga);
+      }
       if (DB) {
         const float2 ph = g.phx[kx];
         d0 = cadd(d0, cmul(ph, DB[eo]));
         d1 = cadd(d1, cmul(ph, DB[eo + TM * A]));
         d2 = cadd(d2, cmul(ph, DB[eo + 2 * TM * A]));
+        if constexpr (CG) {
+          DB[eo] = cmulc(d0, ph);
+          DB[eo + TM * A] = cmulc(d1, ph);
+          DB[eo + 2 * TM * A] = cmulc(d2, ph);
+        }
       }
-      acc += (double)coef[m][3] * ((double)d0.x * d0.x + (double)d0.y * d0.y + (double)d1.x * d1.x +
-                                   (double)d1.y * d1.y + (double)d2.x * d2.x + (double)d2.y * d2.y);
+      const double e2m = (double)d0.x * d0.x + (double)d0.y * d0.y + (double)d1.x * d1.x + (double)d1.y * d1.y +
+                         (double)d2.x * d2.x + (double)d2.y * d2.y;
+      acc += (double)coef[m][3] * e2m;
       if (!do_gamma) continue;
       const float k0 = coef[m][0] * g.sx[kx];
       const float kk1 = coef[m][1] * g.cx[kx];
@@ -368,6 +400,9 @@ __global__ void __launch_bounds__(NT) k_xpass_h(GridDev g, const RhsDesc* __rest
         const float inv = 1.0f / sqrtf(q2);
         const float n0 = k0 * inv, n1 = kk1 * inv, n2 = kk2 * inv;
         const float rhox = (n0 * d0.x + n1 * d1.x + n2 * d2.x) * inv, rhoy = (n0 * d0.y + n1 * d1.y + n2 * d2.y) * inv;
+        if constexpr (CG)
+          acc2 += (double)coef[m][3] *
+                  (e2m / (double)q2 - (double)ccg * ((double)rhox * rhox + (double)rhoy * rhoy));
         const float cA = d.cA * inv * inv, cB = d.cB * inv;
         u0 = make_float2(cA * d0.x + cB * rhox * n0, cA * d0.y + cB * rhoy * n0);
         u1 = make_float2(cA * d1.x + cB * rhox * n1, cA * d1.y + cB * rhoy * n1);
@@ -398,6 +433,10 @@ __global__ void __launch_bounds__(NT) k_xpass_h(GridDev g, const RhsDesc* __rest
   }
   const double bs = block_sum<NT>(acc, red);
   if (threadIdx.x == 0) partials[(long long)r * ntiles + tile] = bs;
+  if constexpr (CG) {
+    const double b2 = block_sum<NT>(acc2, red);
+    if (threadIdx.x == 0) aa.cg_part[(long long)r * ntiles + tile] = b2;
+  }
   if (!do_gamma) return;
   __syncthreads();
 

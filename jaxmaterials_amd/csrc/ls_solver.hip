@@ -253,15 +253,21 @@ struct ls_solver {
   double* aa_gram = nullptr;
   int* aa_head = nullptr;
   int* aa_filled = nullptr;
+  // conjugate gradients (LS_CG; buffers made at the first such solve)
+  int cg = 0;                  // the current solve uses conjugate gradients
+  float* cgP = nullptr;        // search directions [R][6][N]
+  CgState* d_cg = nullptr;     // [R]
+  double* cg_zpart = nullptr;  // [R][nx ny]: z-pass partials of <p, C p>_w
+  double* cg_xpart = nullptr;  // [R][ny P]: x-pass partials of the residual's C0 norm
   // captured chunk of iterations between two host synchronisations (LS_GRAPH)
   hipGraphExec_t gexec = nullptr;
   const void* gkey[3] = {nullptr, nullptr, nullptr};
   double gtol[2] = {0.0, 0.0};
-  int gsteps = 0, gabs = -1;
+  int gsteps = 0, gabs = -1, gcg = -1;
   // timing of the last solve (LS_TIMING set): HIP events on the solver stream
-  static constexpr int kKinds = 6;  // z (plain), y, x, other, z Anderson, Anderson Gram row
-  double t_ms[kKinds] = {0, 0, 0, 0, 0, 0};
-  int t_cnt[kKinds] = {0, 0, 0, 0, 0, 0};
+  static constexpr int kKinds = 7;  // z (plain), y, x, other, z Anderson, Anderson Gram row, z conjugate gradients
+  double t_ms[kKinds] = {0, 0, 0, 0, 0, 0, 0};
+  int t_cnt[kKinds] = {0, 0, 0, 0, 0, 0, 0};
   bool launch_failed = false;  // a pass found no kernel for its tile (set by ypass / xpass)
   int nblk_g = 0;           // blocks per load case of the Gram-row kernel
   long long chunk_g = 0;    // complex entries per block
@@ -415,6 +421,9 @@ AAParams aa_params(const ls_solver* s, int record) {
   aa.filled = s->aa_filled;
   aa.m1 = s->m1;
   aa.record = (record && s->depth > 0) ? 1 : 0;
+  aa.P = s->cgP;
+  aa.cg = s->d_cg;
+  aa.cg_part = s->cg_zpart;
   return aa;
 }
 
@@ -434,7 +443,7 @@ void zline(ls_solver* s, const float* lam, const float* mu, float* eps) {
                                                                  s->tz.lines, s->nblk_z, aa)
   LS_DISPATCH32(s->tz, LS_Z)
 #undef LS_Z
-  mark(s, MODE == Z_AA ? 4 : 0);
+  mark(s, MODE == Z_AA ? 4 : MODE == Z_CG ? 6 : 0);
 }
 
 // y pass over all active load cases; record (inverse only, Anderson): store
@@ -478,20 +487,26 @@ void ypass(ls_solver* s, bool inv, bool record = false) {
 }
 
 // x pass over all active load cases; record: store the residual's spectral
-// form and its Gram row for Anderson (solvers with depth > 0).
-void xpass(ls_solver* s, int do_gamma, bool with_base = true, bool record = false) {
-  const AAParams aa = aa_params(s, record ? 1 : 0);
+// form and its Gram row for Anderson (solvers with depth > 0); cg: the
+// conjugate-gradient form (running divergence in d_base, C0 norm partials).
+void xpass(ls_solver* s, int do_gamma, bool with_base = true, bool record = false, bool cg = false) {
+  AAParams aa = aa_params(s, record ? 1 : 0);
+  aa.cg_part = s->cg_xpart;
+  const int rec = cg ? 2 : aa.record;
   const unsigned grid = (unsigned)s->nblk_x * s->nrhs;
   const size_t lds = s->tx.lds;
-  const float2* db = with_base ? s->dbase : nullptr;
+  float2* db = with_base ? s->dbase : nullptr;
   if (s->tx.reg == 2) {
 #define LS_XH(NV, AV, BV, LV, NTV, IP)                                                                    \
   else if (s->nx == NV && s->tx.lines == LV && s->tx.NT == NTV && s->tx.E == AV && s->tx.inpl == IP) {   \
-    if (aa.record)                                                                                        \
-      k_xpass_h<AV, BV, LV, NTV, true, IP><<<grid, NTV, lds, s->stream>>>(                                 \
+    if (rec == 2)                                                                                         \
+      k_xpass_h<AV, BV, LV, NTV, 2, IP><<<grid, NTV, lds, s->stream>>>(                                    \
+          s->gd, s->d_rhs, s->d_st, s->spec, db, s->partials, s->dc, s->nblk_x, s->dl.per_rhs, do_gamma, aa); \
+    else if (rec == 1)                                                                                    \
+      k_xpass_h<AV, BV, LV, NTV, 1, IP><<<grid, NTV, lds, s->stream>>>(                                    \
           s->gd, s->d_rhs, s->d_st, s->spec, db, s->partials, s->dc, s->nblk_x, s->dl.per_rhs, do_gamma, aa); \
     else                                                                                                  \
-      k_xpass_h<AV, BV, LV, NTV, false, IP><<<grid, NTV, lds, s->stream>>>(                                \
+      k_xpass_h<AV, BV, LV, NTV, 0, IP><<<grid, NTV, lds, s->stream>>>(                                    \
           s->gd, s->d_rhs, s->d_st, s->spec, db, s->partials, s->dc, s->nblk_x, s->dl.per_rhs, do_gamma, aa); \
   }
     if (false) {
@@ -507,12 +522,15 @@ void xpass(ls_solver* s, int do_gamma, bool with_base = true, bool record = fals
   }
   const FastDiv fTm = make_fastdiv(s->tx.lines), fcols = make_fastdiv(6 * s->tx.lines);
 #define LS_X(EV, NTV)                                                                                      \
-  if (aa.record)                                                                                            \
-    k_xpass<EV, NTV, true><<<grid, NTV, lds, s->stream>>>(s->gd, s->d_rhs, s->d_st, s->spec, db, s->partials, s->dc, \
-                                                          fTm, fcols, s->nblk_x, s->dl, do_gamma, aa);         \
+  if (rec == 2)                                                                                             \
+    k_xpass<EV, NTV, 2><<<grid, NTV, lds, s->stream>>>(s->gd, s->d_rhs, s->d_st, s->spec, db, s->partials, s->dc,  \
+                                                       fTm, fcols, s->nblk_x, s->dl, do_gamma, aa);            \
+  else if (rec == 1)                                                                                        \
+    k_xpass<EV, NTV, 1><<<grid, NTV, lds, s->stream>>>(s->gd, s->d_rhs, s->d_st, s->spec, db, s->partials, s->dc,  \
+                                                       fTm, fcols, s->nblk_x, s->dl, do_gamma, aa);            \
   else                                                                                                      \
-    k_xpass<EV, NTV, false><<<grid, NTV, lds, s->stream>>>(s->gd, s->d_rhs, s->d_st, s->spec, db, s->partials,     \
-                                                           s->dc, fTm, fcols, s->nblk_x, s->dl, do_gamma, aa)
+    k_xpass<EV, NTV, 0><<<grid, NTV, lds, s->stream>>>(s->gd, s->d_rhs, s->d_st, s->spec, db, s->partials, s->dc,  \
+                                                       fTm, fcols, s->nblk_x, s->dl, do_gamma, aa)
   LS_DISPATCH32(s->tx, LS_X)
 #undef LS_X
   mark(s, 2);
@@ -521,6 +539,27 @@ void xpass(ls_solver* s, int do_gamma, bool with_base = true, bool record = fals
 void check(ls_solver* s, double rtol, double atol, int mode) {
   k_check<<<s->nrhs, kThreads, 0, s->stream>>>(s->d_st, s->partials, s->dc, s->nblk_x, rtol, atol, mode,
                                                s->absolute);
+  mark(s, 3);
+}
+
+// conjugate-gradient scalars (see CgState in ls_kernels.hpp)
+void cg_alpha(ls_solver* s) {
+  k_cg_alpha<<<s->nrhs, kThreads, 0, s->stream>>>(s->d_st, s->d_cg, s->cg_zpart, s->nblk_z);
+  mark(s, 3);
+}
+void cg_beta(ls_solver* s) {
+  k_cg_beta<<<s->nrhs, kThreads, 0, s->stream>>>(s->d_st, s->d_rhs, s->d_cg, s->cg_xpart, s->nblk_x,
+                                                 1.0 / (double)s->N);
+  mark(s, 3);
+}
+void cg_reset(ls_solver* s, const int* d_sel) {
+  k_cg_reset<<<(s->nrhs + 63) / 64, 64, 0, s->stream>>>(s->d_cg, d_sel, s->nrhs);
+}
+// apply the pending update of every load case, then clear it
+void cg_finish(ls_solver* s, float* delta) {
+  dim3 grid((unsigned)((6 * s->N + kThreads - 1) / kThreads), s->nrhs);
+  k_cg_finish<<<grid, kThreads, 0, s->stream>>>(s->d_cg, s->N, s->cgP, delta);
+  cg_reset(s, nullptr);
   mark(s, 3);
 }
 
@@ -650,7 +689,8 @@ void free_solver(ls_solver* s) {
   }
   void* dev[] = {s->spec,    s->spec64,  s->base,    s->dbase,    s->partials, s->dc,       s->d_rhs,
                  s->d_st,    s->d_sel,   s->part6,   s->d_mean6,  s->d_mm,     s->aaG,      s->aaU,      s->aaPl,
-                 s->aa_alpha, s->aa_alpha_prev, s->aa_dots, s->aa_gram, s->aa_head, s->aa_filled};
+                 s->aa_alpha, s->aa_alpha_prev, s->aa_dots, s->aa_gram, s->aa_head, s->aa_filled,
+                 s->cgP,      s->d_cg,    s->cg_zpart, s->cg_xpart};
   for (void* p : dev) (void)hipFree(p);
   (void)hipHostFree(s->h_st);
   (void)hipHostFree(s->h_mean6);
@@ -863,6 +903,12 @@ void init_fields(ls_solver* s, const int* d_sel, float* delta, const float* src)
 // `advance`, else replacing the newest entry, e.g. after a rebase); the
 // weights follow the inverse y pass of the next iteration (aa_weights).
 void eval_x(ls_solver* s, double rtol, double atol, int mode, bool advance) {
+  if (s->cg) {
+    xpass(s, 1, true, false, true);
+    check(s, rtol, atol, mode);
+    cg_beta(s);
+    return;
+  }
   if (s->depth > 0 && advance)
     k_aa_advance<<<(s->nrhs + 63) / 64, 64, 0, s->stream>>>(s->d_st, s->aa_head, s->aa_filled, s->aa_gram,
                                                            s->aa_alpha, s->aa_alpha_prev, s->nrhs,
@@ -896,6 +942,16 @@ void evaluate(ls_solver* s, double rtol, double atol, int mode) {
 // the x pass of the previous check).
 void iterate(ls_solver* s, const float* lam, const float* mu, float* eps, double rtol, double atol,
              bool evaluate_after) {
+  if (s->cg) {
+    // the z pass applies the previous step's update and forms this step's
+    // direction; alpha makes this step's update pending (applied by the next
+    // z pass or, after the last iteration, by cg_finish)
+    ypass(s, true, false);
+    zline<Z_CG>(s, lam, mu, eps);
+    cg_alpha(s);
+    if (evaluate_after) evaluate(s, rtol, atol, 0);
+    return;
+  }
   ypass(s, true, s->depth > 0);
   if (s->depth == 0) {
     zline<Z_UPD>(s, lam, mu, eps);
@@ -913,7 +969,7 @@ void iterate(ls_solver* s, const float* lam, const float* mu, float* eps, double
 int iterate_graph(ls_solver* s, const float* lam, const float* mu, float* eps, double rtol, double atol,
                   int steps) {
   const bool same = s->gexec && s->gkey[0] == lam && s->gkey[1] == mu && s->gkey[2] == eps && s->gtol[0] == rtol &&
-                    s->gtol[1] == atol && s->gsteps == steps && s->gabs == s->absolute;
+                    s->gtol[1] == atol && s->gsteps == steps && s->gabs == s->absolute && s->gcg == s->cg;
   if (!same) {
     if (s->gexec) {
       LS_HIP(hipGraphExecDestroy(s->gexec));
@@ -933,6 +989,7 @@ int iterate_graph(ls_solver* s, const float* lam, const float* mu, float* eps, d
     s->gtol[1] = atol;
     s->gsteps = steps;
     s->gabs = s->absolute;
+    s->gcg = s->cg;
   }
   LS_HIP(hipGraphLaunch(s->gexec, s->stream));
   return 0;
@@ -941,7 +998,7 @@ int iterate_graph(ls_solver* s, const float* lam, const float* mu, float* eps, d
 // The Anderson iteration keeps delta in its ring (k_aa_delta): write it into
 // its buffer for the selected load cases.
 void aa_write_delta(ls_solver* s, const int* d_sel, float* eps, int last_if_active) {
-  if (s->depth == 0) return;
+  if (s->depth == 0 || s->cg) return;
   dim3 grid((unsigned)((6 * s->N + kThreads - 1) / kThreads), s->nrhs);
   k_aa_delta<<<grid, kThreads, 0, s->stream>>>(s->d_st, d_sel, s->N, s->aaG, s->aa_head, s->aa_filled, s->aa_alpha,
                                                s->aa_alpha_prev, s->m1, last_if_active, eps);
@@ -952,9 +1009,12 @@ void aa_write_delta(ls_solver* s, const int* d_sel, float* eps, int last_if_acti
 int rebase(ls_solver* s, const float* lam, const float* mu, const float* tau, float* eps, double rtol,
            double atol) {
   dim3 grid((unsigned)((6 * s->N + kThreads - 1) / kThreads), s->nrhs);
-  k_rebase<<<grid, kThreads, 0, s->stream>>>(s->d_st, s->N, s->base, eps, s->aaG, s->aa_head, s->aa_filled,
-                                              s->aa_alpha_prev, s->m1);
-  if (s->depth > 0)
+  // conjugate gradients: the pending update first; the recurrence restarts
+  // from the exact divergence of the new base
+  if (s->cg) cg_finish(s, eps);
+  k_rebase<<<grid, kThreads, 0, s->stream>>>(s->d_st, s->N, s->base, eps, s->cg ? nullptr : s->aaG, s->aa_head,
+                                              s->aa_filled, s->aa_alpha_prev, s->m1);
+  if (s->depth > 0 && !s->cg)
     k_aa_forget<<<(s->nrhs + 63) / 64, 64, 0, s->stream>>>(s->d_st, s->aa_alpha_prev, s->nrhs, s->m1);
   base_eval_active(s, lam, mu, tau);
   if (zero_spec(s, active_mask(s))) return -1;
@@ -968,9 +1028,29 @@ int solve_impl(ls_solver* s, const float* d_lambda, const float* d_mu, const int
                int maxiter, double rel_ref, int verbose, int* iters, double* residual, double* sigma_mean,
                bool legacy_q7, int flags = 0) {
   s->absolute = (flags & 1) ? 1 : 0;
+  s->cg = (flags & LS_CG) ? 1 : 0;
   if (maxiter < 0) {
     fprintf(stderr, "liblippmannschwinger: maxiter must be >= 0\n");
     return -1;
+  }
+  if (s->cg) {
+    if (s->gd.dmg) {
+      fprintf(stderr, "liblippmannschwinger: conjugate gradients need a linear material law (damage is set)\n");
+      return -1;
+    }
+    if (!s->cgP) {
+      if (s->arena_mode) {
+        fprintf(stderr, "liblippmannschwinger: conjugate gradients are not available without allocation\n");
+        return -1;
+      }
+      if (dalloc(&s->cgP, (size_t)s->nrhs * 6 * s->N) || dalloc(&s->d_cg, s->nrhs) ||
+          dalloc(&s->cg_zpart, (size_t)s->nrhs * s->nx * s->ny) ||
+          dalloc(&s->cg_xpart, (size_t)s->nrhs * s->ny * s->P))
+        return -1;
+    }
+    // no direction yet (the first beta is 0, and 0 * p must be 0)
+    LS_HIP(hipMemsetAsync(s->cgP, 0, (size_t)s->nrhs * 6 * s->N * sizeof(float), s->stream));
+    cg_reset(s, nullptr);
   }
   for (auto e : s->tev) (void)hipEventDestroy(e);
   s->tev.clear();
@@ -1091,6 +1171,7 @@ int solve_impl(ls_solver* s, const float* d_lambda, const float* d_mu, const int
       init_fields(s, s->d_sel, d_eps, nullptr);
       base_eval_start(s, d_lambda, d_mu, d_tau);
       if (zero_spec(s, restart)) return -1;
+      if (s->cg) cg_reset(s, s->d_sel);
       eval_x(s, drtol, datol, 0, false);
       if (sync_state(s)) return -1;
       for (int r = 0; r < s->nrhs; ++r)
@@ -1169,6 +1250,7 @@ int solve_impl(ls_solver* s, const float* d_lambda, const float* d_mu, const int
       for (int r = 0; r < s->nrhs; ++r) sel[r] = q7[r] ? 0 : 1;
     if (put_sel(s, sel)) return -1;
     aa_write_delta(s, nullptr, d_eps, 1);
+    if (s->cg) cg_finish(s, d_eps);
     k_stress<<<grid, kThreads, 0, s->stream>>>(s->gd, s->d_rhs, s->d_sel, d_lambda, d_mu, s->base, d_eps, 1, d_tau,
                                                 d_sigma, s->part6, s->chunk_stress);
     if (q7_done) {

@@ -18,6 +18,8 @@ reference file and lines it follows (paths relative to the reference repo):
 * ``lippmann_schwinger_jax``     src/jaxmaterials/solver/lippmann_schwinger.py:146-265
 * ``lippmann_schwinger_capi``    cuda/src/lippmann_schwinger.cu:166-234 (loop and
                                  return semantics of the C ABI)
+* ``lippmann_schwinger_cg``      conjugate gradients on the same equation (no reference
+                                 counterpart; Krylov-Galerkin optimality pinned on a dense grid)
 * ``effective_tensor``           src/jaxmaterials/data/data.py:178-186
 * ``material_properties``        src/jaxmaterials/data/data.py:267-314
 
@@ -444,6 +446,65 @@ def lippmann_schwinger_capi(lmbda, mu, epsilon_bar, grid_spec, rtol, atol, maxit
     else:
         it = maxiter
     return eps, sigma, it
+
+
+def c0_dot(a, b, lmbda0, mu0):
+    """<a, C0 b>_w = sum_x 2 mu0 sum_i w_i a_i b_i + lmbda0 tr a tr b (isotropic reference medium)."""
+    return float(2.0 * mu0 * np.einsum("aijk,aijk,a->", a, b, VOIGT_WEIGHTS)
+                 + lmbda0 * np.sum((a[0] + a[1] + a[2]) * (b[0] + b[1] + b[2])))
+
+
+def lippmann_schwinger_cg(lmbda, mu, epsilon_bar, grid_spec, rtol=1e-6, atol=0.0, maxiter=1000, workers=None,
+                          epsilon0=None, rel_error_ref=None, callback=None):
+    """Conjugate gradients on the Lippmann-Schwinger equation (no counterpart in
+    the reference, which iterates the basic scheme or Anderson mixing,
+    lippmann_schwinger.py:146-265): textbook CG (Hestenes & Stiefel 1952) for
+    A e = b on compatible zero-mean strains, A = Gamma0 C, b = -Gamma0 C eps_bar,
+    in the reference-medium inner product <a, b>_C0 (in which A is self-adjoint
+    and positive definite; Gamma0 C0 is the C0-orthogonal projection onto
+    compatible fields).  The residual b - A e of eps = eps_bar + e is the
+    Lippmann-Schwinger update -Gamma0 (C eps) (fourier_solve).  Same reference
+    medium, residual R of the iterate's stress, stop test and update count as
+    lippmann_schwinger_rfft; float64, real FFTs.  Returns (epsilon, sigma, iter,
+    history of R).
+    """
+    sp = _RealSpectral(grid_spec, workers)
+    lmbda = np.asarray(lmbda, dtype=np.float64)
+    mu = np.asarray(mu, dtype=np.float64)
+    mu0, lmbda0 = 0.5 * (mu.min() + mu.max()), 0.5 * (lmbda.min() + lmbda.max())
+    if epsilon0 is None:
+        eps = np.broadcast_to(np.asarray(epsilon_bar, dtype=np.float64)[:, None, None, None],
+                              (6,) + _shape(grid_spec)).copy()
+    else:
+        eps = np.asarray(epsilon0, dtype=np.float64).copy()
+
+    def update(sigma_hat):  # -Gamma0 sigma_hat in real space
+        return sp.inv(fourier_solve(sigma_hat, lmbda0, mu0, sp.xizero))
+
+    sigma = compute_sigma(lmbda, mu, eps.copy())
+    sigma_hat = sp.fwd(sigma)
+    R, _ = sp.residual(sigma_hat)
+    R0 = R if rel_error_ref is None else rel_error_ref
+    history = [R]
+    r = update(sigma_hat)
+    p = r.copy()
+    rr = c0_dot(r, r, lmbda0, mu0)
+    it = 0
+    while (R > atol) and (R > rtol * R0) and (it < maxiter):
+        q = -update(sp.fwd(compute_sigma(lmbda, mu, p.copy())))  # A p
+        alpha = rr / c0_dot(p, q, lmbda0, mu0)
+        eps += alpha * p
+        r -= alpha * q
+        rr_new = c0_dot(r, r, lmbda0, mu0)
+        p = r + (rr_new / rr) * p
+        rr = rr_new
+        sigma = compute_sigma(lmbda, mu, eps.copy())
+        R, _ = sp.residual(sp.fwd(sigma))
+        history.append(R)
+        it += 1
+        if callback is not None:
+            callback(it, R)
+    return eps, sigma, it, history
 
 
 def effective_tensor(lmbda, mu, grid_spec, rtol=1e-6, atol=0.0, depth=0, maxiter=2000,

@@ -424,3 +424,53 @@ def test_ceff_gradient_closed_form_matches_finite_differences():
                 mm[v] -= h
             fd = float((W * (ceff(lp, mp) - ceff(lm_, mm))).sum()) / (2 * h)
             assert abs(grad[v] - fd) <= 1e-6 * max(abs(fd), 1e-3), (field, v, grad[v], fd)
+
+
+# ------------------------------------------------------------------ conjugate gradients
+def _energy(x, lm, mu):
+    return float(np.einsum("aijk,aijk,a->", x, O.compute_sigma(lm, mu, x.copy()), O.VOIGT_WEIGHTS))
+
+
+@pytest.mark.parametrize("dims", [(4, 4, 4), (5, 4, 3)])
+def test_cg_iterates_are_krylov_galerkin_optimal(dims):
+    """The k-th CG iterate minimises the energy norm of the error
+    <e - e*, C (e - e*)>_w over eps_bar + span{r0, A r0, ..., A^(k-1) r0},
+    A = Gamma0 C (the defining property of CG in the C0 product, in which A is
+    self-adjoint): the minimiser is found by a dense least-squares solve over
+    an explicit Krylov basis, independent of CG's recurrences (a wrong alpha
+    or beta, or the plain product in place of C0, breaks it); e* comes from
+    the dense brute-force solve of the discrete operator."""
+    gs = O.GridSpec(*dims, 1.0, 1.3, 0.8)
+    rng = np.random.default_rng(sum(dims) + 11)
+    lm, mu = material(dims, rng, 1.0, 8.0)
+    eb = rng.normal(size=6)
+    estar = dense_solution(lm, mu, eb, gs) - eb[:, None, None, None]
+    sp = O._RealSpectral(gs)
+    l0, m0 = 0.5 * (lm.min() + lm.max()), 0.5 * (mu.min() + mu.max())
+    update = lambda s: sp.inv(O.fourier_solve(sp.fwd(s), l0, m0, sp.xizero))  # noqa: E731
+    ebf = np.broadcast_to(eb[:, None, None, None], (6,) + gs.shape).copy()
+    r0 = update(O.compute_sigma(lm, mu, ebf.copy()))
+    basis = [r0]
+    for k in range(1, 5):
+        eps_k = O.lippmann_schwinger_cg(lm, mu, eb, gs, rtol=0.0, maxiter=k)[0] - ebf
+        # energy-optimal combination of the basis (Gram system in the energy product)
+        G = np.array([[_energy(a + b, lm, mu) - _energy(a - b, lm, mu) for b in basis] for a in basis]) / 4
+        rhs = np.array([(_energy(a + estar, lm, mu) - _energy(a - estar, lm, mu)) / 4 for a in basis])
+        y = np.linalg.solve(G, rhs)
+        best = sum(c * v for c, v in zip(y, basis))
+        assert np.abs(eps_k - best).max() <= 1e-8 * np.abs(estar).max(), k
+        basis.append(-update(O.compute_sigma(lm, mu, basis[-1].copy())))  # A^k r0
+
+
+def test_cg_converges_to_the_plain_solution_and_counts_updates():
+    gs = py_grid("odd")
+    rng = np.random.default_rng(784173)
+    lm, mu = material(gs.shape, rng)
+    eb = np.array([1.0, 0.4, 0.3, 0.1, -0.4, 0.7])
+    e_cg, s_cg, it, hist = O.lippmann_schwinger_cg(lm, mu, eb, gs, rtol=1e-12, maxiter=200)
+    e_pl, _, _, _ = O.lippmann_schwinger_rfft(lm, mu, eb, gs, rtol=1e-12, depth=0, maxiter=2000)
+    assert np.abs(e_cg - e_pl).max() <= 1e-9 * np.abs(e_pl).max()
+    assert len(hist) == it + 1 and hist[-1] <= 1e-12 * hist[0] < hist[-2]
+    # faster than the basic scheme on the reference test medium
+    _, _, it_pl, _ = O.lippmann_schwinger_rfft(lm, mu, eb, gs, rtol=1e-6, depth=0, maxiter=2000)
+    assert O.lippmann_schwinger_cg(lm, mu, eb, gs, rtol=1e-6, maxiter=200)[2] < it_pl
