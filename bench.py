@@ -442,10 +442,9 @@ def main():
                 net = WarmStartNet.load(DEFAULT_WEIGHTS)
             r_init = warm_start_solve(net, solver, lam, mu, loads, epsilon=eps, rtol=args.rtol, maxiter=0,
                                       check=False, cg=use_cg)["residual"]
-            eb6 = torch.eye(6, device="cuda")
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            net.forward(lam.reshape((1,) + shape), mu.reshape((1,) + shape), eb6, 6, eps)
+            net.forward(lam.reshape((1,) + shape), mu.reshape((1,) + shape), eps, extents=ext)
             torch.cuda.synchronize()
             ws_info = {"initial_residual_vs_uniform": [float(v) for v in r_init],
                        "inference_ms": 1e3 * (time.perf_counter() - t0),

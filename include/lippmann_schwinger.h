@@ -74,11 +74,15 @@ ls_solver *ls_solver_create(const int *voxels, const float *extents, int nrhs,
  * gradients instead of the Lippmann-Schwinger / Anderson update (the same
  * equation, residual, stop test and iteration count semantics: Gamma0 C is
  * self-adjoint and positive definite in the reference medium's inner
- * product on compatible fields; needs Hooke's law, any depth).  Returns 0 if
+ * product on compatible fields; needs Hooke's law, any depth);
+ * LS_COMPATIBLE_START = the warm start in d_eps is compatible with mean
+ * eps_bar already (e.g. eps_bar + sym(D^+ u) on the rotated grid, as the
+ * learned warm start writes it), so its projection is skipped.  Returns 0 if
  * all converged, 1 if some did not, -1 on error. */
 #define LS_ABSOLUTE_RESIDUAL 1
 #define LS_GRAPH 2
 #define LS_CG 4
+#define LS_COMPATIBLE_START 8
 int ls_solver_solve(ls_solver *s, const float *d_lambda, const float *d_mu,
                     const int *rhs_mat, const float *eps_bar,
                     const float *d_tau, float *d_eps, float *d_sigma,

@@ -87,6 +87,42 @@ int ls_ws_mse(const float *pred, const uint16_t *target, int B, long long nper,
               float gscale, uint16_t *grad, double *part, long long part_cap,
               double *loss, void *stream);
 
+/* Compatible strains from predicted displacements: per load case k of S
+ * samples, eps_k = sym(D^+ u_k), D^+ = -(D^-)^T the forward derivative of the
+ * rotated staggered grid (D^+_d = i xi~_d times a phase common to the axes
+ * in Fourier, so eps lies in the range of the solver's Gamma0 and has zero
+ * mean).  dims = {S, nx, ny, nz}; u fp32 [S][18][nper] (3 k + c); eps fp32
+ * [S][36][nper] (6 k + c, Voigt with tensor shear), plus e_k if add_unit. */
+int ls_ws_symgrad(const int *dims, const float *extents, const float *u,
+                  int add_unit, float *eps, void *stream);
+
+/* Adjoint of ls_ws_symgrad: grad_u bf16 [S][nper][ldu] (channel 3 k + c,
+ * channels >= 18 zero) from grad_eps bf16 [S][nper][ldg] (channel 6 k + c). */
+int ls_ws_symgrad_backward(const int *dims, const float *extents,
+                           const uint16_t *grad_eps, int ldg,
+                           uint16_t *grad_u, int ldu, void *stream);
+
+/* Training loss on the solver's own terms for S samples of the six unit load
+ * cases: dims = {S, nx, ny, nz}, extents = {Lx, Ly, Lz}; pred fp32 and target
+ * bf16 [S][36][nper] (strain fluctuation, channel 6 k + c = component c of
+ * load case k); lam, mu fp32 [S][nper].  With e = pred - target,
+ *   L = sum_{s,k} [ wm <e, C e>_w / <t, C t>_w + wr |D^- . C e|^2 / |D^- . C e_k|^2 ] / (6 S):
+ * the relative strain energy of the error (the norm conjugate gradients
+ * reduce) and the squared equilibrium residual of the predicted strain
+ * relative to the uniform start's (the solver's stop measure; the target is
+ * in equilibrium), D^- the backward divergence of the rotated staggered grid
+ * (src/jaxmaterials/solver/derivatives.py:8-96).  grad bf16 [S][nper][ldg] =
+ * dL/dpred (ldg >= 36, a multiple of 8; channels >= 36 zero).  scratch fp32
+ * of >= 54 S nper floats, part FP64 of >= 24 S ceil(nper / 256), scale fp32
+ * [S][6][2] (the two gradient factors), loss FP64 [1] (device). */
+int ls_ws_residual_loss(const int *dims, const float *extents,
+                        const float *pred, const uint16_t *target,
+                        const float *lam, const float *mu, float wm,
+                        float wr, uint16_t *grad, int ldg, float *scratch,
+                        long long scratch_cap, double *part,
+                        long long part_cap, float *scale, double *loss,
+                        void *stream);
+
 /* Adam update of n fp32 parameters (step >= 1, bias-corrected). */
 int ls_ws_adam(float *w, const float *g, float *m, float *v, long long n,
                float lr, float b1, float b2, float eps, int step, void *stream);

@@ -478,6 +478,376 @@ __global__ void __launch_bounds__(kThreads) k_mse(const float* __restrict__ pred
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
+// ---------------------------------------------------------------------------
+// Training loss on the solver's own terms (ls_ws_residual_loss).  Per sample
+// s and unit load case k (K = 6) with the prediction error e = pred - target
+// (the fluctuation, six Voigt components):
+//   L = sum_{s,k} [ wm <e, C e>_w / <t, C t>_w + wr |D^- . C e|^2 / |D^- . C e_k|^2 ] / (S K),
+// C the sample's Hooke law, <., .>_w the Voigt-weighted product (so <e, C e>_w
+// is the strain energy of the error: the norm conjugate gradients reduce), D^-
+// the backward divergence of the rotated staggered grid (the 2 x 2 x 2 cell of
+// voxels (i - a, j - b, k - c)).  The target is an equilibrium field
+// (D^- . C (e_k + t) = 0 to the solver's tolerance), so the second term is the
+// squared equilibrium residual of the predicted strain relative to the uniform
+// start's: the solver's stop measure (equal to its Fourier residual by
+// Parseval).  Partials are FP64 per block in a fixed order.  Channel index
+// ch = 6 k + c.
+constexpr int kRlK = 6;
+struct RlGeo {
+  int S, X, Y, Z;
+  long long nper;
+  float sx, sy, sz;  // n_d / (4 L_d)
+};
+
+__device__ __forceinline__ void rl_ijk(const RlGeo& g, long long v, int& i, int& j, int& k) {
+  k = (int)(v % g.Z);
+  const long long q = v / g.Z;
+  j = (int)(q % g.Y);
+  i = (int)(q / g.Y);
+}
+
+// stress of the error, sigma_e = C e, into sc [S][36][nper]; partials of <e, C e>_w and <t, C t>_w per k
+__global__ void __launch_bounds__(kThreads) k_rl_stress(RlGeo g, const float* __restrict__ pred,
+                                                        const uint16_t* __restrict__ tgt,
+                                                        const float* __restrict__ lam, const float* __restrict__ mu,
+                                                        float* __restrict__ sc, double* __restrict__ part) {
+  __shared__ double red[2 * kRlK][kThreads / 64];
+  const int s = blockIdx.y;
+  const long long v = (long long)blockIdx.x * kThreads + threadIdx.x;
+  double acc[2 * kRlK];
+#pragma unroll
+  for (int q = 0; q < 2 * kRlK; ++q) acc[q] = 0.0;
+  if (v < g.nper) {
+    const float l = lam[(long long)s * g.nper + v], m2 = 2.f * mu[(long long)s * g.nper + v];
+#pragma unroll
+    for (int k = 0; k < kRlK; ++k) {
+      float e[6], t[6];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        const size_t idx = ((size_t)s * 36 + 6 * k + c) * g.nper + v;
+        t[c] = bf2f(tgt[idx]);
+        e[c] = pred[idx] - t[c];
+      }
+      const float tr = e[0] + e[1] + e[2], trt = t[0] + t[1] + t[2];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        const float se = m2 * e[c] + (c < 3 ? l * tr : 0.f), st = m2 * t[c] + (c < 3 ? l * trt : 0.f);
+        const double wc = c < 3 ? 1.0 : 2.0;
+        acc[2 * k] += wc * (double)e[c] * se;
+        acc[2 * k + 1] += wc * (double)t[c] * st;
+        sc[((size_t)s * 36 + 6 * k + c) * g.nper + v] = se;
+      }
+    }
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < 2 * kRlK; ++q) {
+    double a = acc[q];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+    if (lane == 0) red[q][w] = a;
+  }
+  __syncthreads();
+  if (threadIdx.x < 2 * kRlK) {
+    double a = 0.0;
+    for (int i = 0; i < kThreads / 64; ++i) a += red[threadIdx.x][i];
+    part[((size_t)s * gridDim.x + blockIdx.x) * (4 * kRlK) + threadIdx.x] = a;
+  }
+}
+
+// divergence of the error stress, dv = D^- . sigma_e [S][K][3][nper], and of the
+// uniform start's stress C e_k; partials of |dv|^2 and |D^- . C e_k|^2 per k
+__global__ void __launch_bounds__(kThreads) k_rl_div(RlGeo g, const float* __restrict__ sc,
+                                                     const float* __restrict__ lam, const float* __restrict__ mu,
+                                                     float* __restrict__ dv, double* __restrict__ part) {
+  __shared__ double red[2 * kRlK][kThreads / 64];
+  const int s = blockIdx.y;
+  const long long v = (long long)blockIdx.x * kThreads + threadIdx.x;
+  double acc[2 * kRlK];
+#pragma unroll
+  for (int q = 0; q < 2 * kRlK; ++q) acc[q] = 0.0;
+  if (v < g.nper) {
+    int i, j, k;
+    rl_ijk(g, v, i, j, k);
+    const int im = i == 0 ? g.X - 1 : i - 1, jm = j == 0 ? g.Y - 1 : j - 1, km = k == 0 ? g.Z - 1 : k - 1;
+    const long long ii[2] = {(long long)i * g.Y * g.Z, (long long)im * g.Y * g.Z};
+    const long long jj[2] = {(long long)j * g.Z, (long long)jm * g.Z};
+    const int kk[2] = {k, km};
+    // D^-_d of one field f (offset base) over the cell
+    auto dd = [&](const float* f, float& dx, float& dy, float& dz) {
+      float c[2][2][2];
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+          for (int e = 0; e < 2; ++e) c[a][b][e] = f[ii[a] + jj[b] + kk[e]];
+      float tx = 0.f, ty = 0.f, tz = 0.f;
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          tx += c[0][a][b] - c[1][a][b];
+          ty += c[a][0][b] - c[a][1][b];
+          tz += c[a][b][0] - c[a][b][1];
+        }
+      dx = g.sx * tx;
+      dy = g.sy * ty;
+      dz = g.sz * tz;
+    };
+    const float* L = lam + (long long)s * g.nper;
+    const float* Mu = mu + (long long)s * g.nper;
+    float lx, ly, lz, mx, my, mz;
+    dd(L, lx, ly, lz);
+    dd(Mu, mx, my, mz);
+#pragma unroll
+    for (int q = 0; q < kRlK; ++q) {
+      const float* base = sc + ((size_t)s * 36 + 6 * q) * g.nper;
+      float d0[6][3];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) dd(base + (size_t)c * g.nper, d0[c][0], d0[c][1], d0[c][2]);
+      // div_x = D_x s_xx + D_y s_xy + D_z s_xz, div_y = D_x s_xy + D_y s_yy + D_z s_yz,
+      // div_z = D_x s_xz + D_y s_yz + D_z s_zz (Voigt xx yy zz xy xz yz)
+      const float q0 = d0[0][0] + d0[3][1] + d0[4][2];
+      const float q1 = d0[3][0] + d0[1][1] + d0[5][2];
+      const float q2 = d0[4][0] + d0[5][1] + d0[2][2];
+      float* o = dv + ((size_t)s * kRlK + q) * 3 * g.nper + v;
+      o[0] = q0;
+      o[g.nper] = q1;
+      o[2 * g.nper] = q2;
+      acc[2 * q] += (double)q0 * q0 + (double)q1 * q1 + (double)q2 * q2;
+      // the uniform start's stress C e_q: 2 mu e_q + lambda tr(e_q) I
+      float u0, u1, u2;
+      if (q < 3) {
+        const float dl[3] = {lx, ly, lz}, dm[3] = {mx, my, mz};
+        u0 = dl[0] + (q == 0 ? 2.f * dm[0] : 0.f);
+        u1 = dl[1] + (q == 1 ? 2.f * dm[1] : 0.f);
+        u2 = dl[2] + (q == 2 ? 2.f * dm[2] : 0.f);
+      } else {
+        // shear e_q (tensor component 1 in slot q): sigma_q = 2 mu
+        const float dm[3] = {mx, my, mz};
+        const int a = q == 3 ? 0 : (q == 4 ? 0 : 1), b = q == 3 ? 1 : (q == 4 ? 2 : 2);
+        float w3[3] = {0.f, 0.f, 0.f};
+        w3[a] = 2.f * dm[b];
+        w3[b] = 2.f * dm[a];
+        u0 = w3[0];
+        u1 = w3[1];
+        u2 = w3[2];
+      }
+      acc[2 * q + 1] += (double)u0 * u0 + (double)u1 * u1 + (double)u2 * u2;
+    }
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < 2 * kRlK; ++q) {
+    double a = acc[q];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+    if (lane == 0) red[q][w] = a;
+  }
+  __syncthreads();
+  if (threadIdx.x < 2 * kRlK) {
+    double a = 0.0;
+    for (int i = 0; i < kThreads / 64; ++i) a += red[threadIdx.x][i];
+    part[((size_t)s * gridDim.x + blockIdx.x) * (4 * kRlK) + 2 * kRlK + threadIdx.x] = a;
+  }
+}
+
+// per (s, k): the two scale factors of the gradient, and the loss (one block)
+__global__ void __launch_bounds__(kThreads) k_rl_reduce(int S, int nblk, float wm, float wr,
+                                                        const double* __restrict__ part,
+                                                        float* __restrict__ scale, double* __restrict__ loss) {
+  __shared__ double sums[64][4 * kRlK];
+  __shared__ double tot;
+  const int nq = 4 * kRlK;
+  if (threadIdx.x == 0) tot = 0.0;
+  __syncthreads();
+  for (int s0 = 0; s0 < S; s0 += 64) {
+    const int ns = min(64, S - s0);
+    for (int t = threadIdx.x; t < ns * nq; t += kThreads) {
+      const int sl = t / nq, q = t - sl * nq;
+      double a = 0.0;
+      for (int b = 0; b < nblk; ++b) a += part[((size_t)(s0 + sl) * nblk + b) * nq + q];
+      sums[sl][q] = a;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const double inv = 1.0 / ((double)S * kRlK);
+      for (int sl = 0; sl < ns; ++sl)
+        for (int k = 0; k < kRlK; ++k) {
+          const double E = sums[sl][2 * k], T = fmax(sums[sl][2 * k + 1], 1e-30);
+          const double Q = sums[sl][2 * kRlK + 2 * k], D = fmax(sums[sl][2 * kRlK + 2 * k + 1], 1e-30);
+          tot += inv * (wm * E / T + wr * Q / D);
+          scale[((s0 + sl) * kRlK + k) * 2] = (float)(2.0 * wm * inv / T);
+          scale[((s0 + sl) * kRlK + k) * 2 + 1] = (float)(2.0 * wr * inv / D);
+        }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) loss[0] = tot;
+}
+
+// dL/dpred = cm w C e + cr C (D^-)^T dv (w the Voigt weights), with (D^-_x)^T f (x) = sx sum_{b,c} f(i, j + b, k + c) - f(i + 1, j + b, k + c)
+// (and likewise along y and z); into grad bf16 [S][nper][ldg], channels >= 36 zero
+__global__ void __launch_bounds__(kThreads) k_rl_grad(RlGeo g, const float* __restrict__ sc,
+                                                      const float* __restrict__ lam, const float* __restrict__ mu,
+                                                      const float* __restrict__ dv, const float* __restrict__ scale,
+                                                      uint16_t* __restrict__ grad, int ldg) {
+  const int s = blockIdx.y;
+  const long long v = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (v >= g.nper) return;
+  int i, j, k;
+  rl_ijk(g, v, i, j, k);
+  const int ip = i + 1 == g.X ? 0 : i + 1, jp = j + 1 == g.Y ? 0 : j + 1, kp = k + 1 == g.Z ? 0 : k + 1;
+  const long long ii[2] = {(long long)i * g.Y * g.Z, (long long)ip * g.Y * g.Z};
+  const long long jj[2] = {(long long)j * g.Z, (long long)jp * g.Z};
+  const int kk[2] = {k, kp};
+  auto dt = [&](const float* f, float& dx, float& dy, float& dz) {
+    float c[2][2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int e = 0; e < 2; ++e) c[a][b][e] = f[ii[a] + jj[b] + kk[e]];
+    float tx = 0.f, ty = 0.f, tz = 0.f;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        tx += c[0][a][b] - c[1][a][b];
+        ty += c[a][0][b] - c[a][1][b];
+        tz += c[a][b][0] - c[a][b][1];
+      }
+    dx = g.sx * tx;
+    dy = g.sy * ty;
+    dz = g.sz * tz;
+  };
+  const float l = lam[(long long)s * g.nper + v], m2 = 2.f * mu[(long long)s * g.nper + v];
+  uint16_t* o = grad + ((size_t)s * g.nper + v) * ldg;
+  for (int q = 0; q < kRlK; ++q) {
+    const float* f = dv + ((size_t)s * kRlK + q) * 3 * g.nper;
+    float a[3][3];  // a[i][d] = (D^-_d)^T dv_i
+#pragma unroll
+    for (int c = 0; c < 3; ++c) dt(f + (size_t)c * g.nper, a[c][0], a[c][1], a[c][2]);
+    // d/dsigma (Voigt): xx: a_x,x; yy: a_y,y; zz: a_z,z; xy: a_x,y + a_y,x; xz: a_x,z + a_z,x; yz: a_y,z + a_z,y
+    float gs[6] = {a[0][0], a[1][1], a[2][2], a[0][1] + a[1][0], a[0][2] + a[2][0], a[1][2] + a[2][1]};
+    const float trg = gs[0] + gs[1] + gs[2];
+    const float cm = scale[(s * kRlK + q) * 2], cr = scale[(s * kRlK + q) * 2 + 1];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      const float se = sc[((size_t)s * 36 + 6 * q + c) * g.nper + v];  // (C e)_c
+      const float ge = m2 * gs[c] + (c < 3 ? l * trg : 0.f);          // C (symmetric) applied to the stress gradient
+      o[6 * q + c] = f2bf(cm * (c < 3 ? 1.f : 2.f) * se + cr * ge);
+    }
+  }
+  for (int c = 36; c < ldg; ++c) o[c] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Compatible strains from predicted displacements (ls_ws_symgrad): per load
+// case k, eps_k = sym(D^+ u_k) with D^+_d = -(D^-_d)^T the forward derivative
+// of the rotated staggered grid,
+//   D^+_x f(i, j, k) = sx sum_{b,c in {0,1}} f(i + 1, j + b, k + c) - f(i, j + b, k + c)
+// (and likewise along y, z), Voigt xx yy zz xy xz yz with tensor shear.  In
+// Fourier D^+_d is i xi~_d times a phase common to the three axes, so
+// sym(D^+ u) lies in the range of the solver's Gamma0: the start needs no
+// projection.  u fp32 [S][18][nper] (3 k + c), eps fp32 [S][36][nper]
+// (6 k + c), plus e_k if add_unit.
+__global__ void __launch_bounds__(kThreads) k_symgrad(RlGeo g, const float* __restrict__ u, int add_unit,
+                                                      float* __restrict__ eps) {
+  const int s = blockIdx.y;
+  const long long v = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (v >= g.nper) return;
+  int i, j, k;
+  rl_ijk(g, v, i, j, k);
+  const int ip = i + 1 == g.X ? 0 : i + 1, jp = j + 1 == g.Y ? 0 : j + 1, kp = k + 1 == g.Z ? 0 : k + 1;
+  const long long ii[2] = {(long long)i * g.Y * g.Z, (long long)ip * g.Y * g.Z};
+  const long long jj[2] = {(long long)j * g.Z, (long long)jp * g.Z};
+  const int kk[2] = {k, kp};
+  for (int q = 0; q < kRlK; ++q) {
+    float d[3][3];  // d[c][dir] = D^+_dir u_c
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const float* f = u + ((size_t)s * 18 + 3 * q + c) * g.nper;
+      float cell[2][2][2];
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+          for (int e = 0; e < 2; ++e) cell[a][b][e] = f[ii[a] + jj[b] + kk[e]];
+      float tx = 0.f, ty = 0.f, tz = 0.f;
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          tx += cell[1][a][b] - cell[0][a][b];
+          ty += cell[a][1][b] - cell[a][0][b];
+          tz += cell[a][b][1] - cell[a][b][0];
+        }
+      d[c][0] = g.sx * tx;
+      d[c][1] = g.sy * ty;
+      d[c][2] = g.sz * tz;
+    }
+    float e[6] = {d[0][0], d[1][1], d[2][2], 0.5f * (d[0][1] + d[1][0]), 0.5f * (d[0][2] + d[2][0]),
+                  0.5f * (d[1][2] + d[2][1])};
+    if (add_unit) e[q] += 1.f;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) eps[((size_t)s * 36 + 6 * q + c) * g.nper + v] = e[c];
+  }
+}
+
+// Adjoint: dL/du from dL/deps (bf16 [S][nper][ldg], channel 6 k + c), with
+// (D^+_d)^T = -D^-_d: dL/du_x = -(D^-_x g_xx + D^-_y g_xy / 2 + D^-_z g_xz / 2), etc.
+// Into bf16 [S][nper][ldu] (channel 3 k + c; channels >= 18 zero).
+__global__ void __launch_bounds__(kThreads) k_symgrad_back(RlGeo g, const uint16_t* __restrict__ ge, int ldg,
+                                                           uint16_t* __restrict__ gu, int ldu) {
+  const int s = blockIdx.y;
+  const long long v = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (v >= g.nper) return;
+  int i, j, k;
+  rl_ijk(g, v, i, j, k);
+  const int im = i == 0 ? g.X - 1 : i - 1, jm = j == 0 ? g.Y - 1 : j - 1, km = k == 0 ? g.Z - 1 : k - 1;
+  const long long ii[2] = {(long long)i * g.Y * g.Z, (long long)im * g.Y * g.Z};
+  const long long jj[2] = {(long long)j * g.Z, (long long)jm * g.Z};
+  const int kk[2] = {k, km};
+  const uint16_t* G = ge + (size_t)s * g.nper * ldg;
+  uint16_t* o = gu + ((size_t)s * g.nper + v) * ldu;
+  for (int q = 0; q < kRlK; ++q) {
+    float dm[6][3];  // D^-_dir of the six components of dL/deps_k
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      float cell[2][2][2];
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+          for (int e = 0; e < 2; ++e) cell[a][b][e] = bf2f(G[(size_t)(ii[a] + jj[b] + kk[e]) * ldg + 6 * q + c]);
+      float tx = 0.f, ty = 0.f, tz = 0.f;
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          tx += cell[0][a][b] - cell[1][a][b];
+          ty += cell[a][0][b] - cell[a][1][b];
+          tz += cell[a][b][0] - cell[a][b][1];
+        }
+      dm[c][0] = g.sx * tx;
+      dm[c][1] = g.sy * ty;
+      dm[c][2] = g.sz * tz;
+    }
+    const float ux = -(dm[0][0] + 0.5f * dm[3][1] + 0.5f * dm[4][2]);
+    const float uy = -(0.5f * dm[3][0] + dm[1][1] + 0.5f * dm[5][2]);
+    const float uz = -(0.5f * dm[4][0] + 0.5f * dm[5][1] + dm[2][2]);
+    o[3 * q] = f2bf(ux);
+    o[3 * q + 1] = f2bf(uy);
+    o[3 * q + 2] = f2bf(uz);
+  }
+  for (int c = 18; c < ldu; ++c) o[c] = 0;
+}
+
 __global__ void __launch_bounds__(kThreads) k_sum_f64(const double* __restrict__ part, int n, double* __restrict__ out) {
   __shared__ double red[kThreads];
   double s = 0.0;
@@ -530,6 +900,21 @@ __global__ void __launch_bounds__(kThreads) k_pack(const float* __restrict__ w, 
     }
     wd[u] = f2bf(val);
   }
+}
+
+bool rl_geo(const int* dims, const float* extents, RlGeo& g) {
+  if (!dims || !extents || dims[0] < 1 || dims[1] < 1 || dims[2] < 1 || dims[3] < 1 ||
+      !(extents[0] > 0.f && extents[1] > 0.f && extents[2] > 0.f))
+    return false;
+  g.S = dims[0];
+  g.X = dims[1];
+  g.Y = dims[2];
+  g.Z = dims[3];
+  g.nper = (long long)g.X * g.Y * g.Z;
+  g.sx = (float)(0.25 * g.X / (double)extents[0]);
+  g.sy = (float)(0.25 * g.Y / (double)extents[1]);
+  g.sz = (float)(0.25 * g.Z / (double)extents[2]);
+  return true;
 }
 
 }  // namespace wsk
@@ -764,6 +1149,55 @@ int ls_ws_mse(const float* pred, const uint16_t* target, int B, long long nper, 
   k_mse<<<nb, kThreads, 0, st>>>(pred, target, B, nper, gscale, grad, part);
   k_sum_f64<<<1, kThreads, 0, st>>>(part, (int)nb, loss);
   return launch_ok("mse");
+}
+
+
+int ls_ws_symgrad(const int* dims, const float* extents, const float* u, int add_unit, float* eps, void* stream) {
+  RlGeo g;
+  if (!rl_geo(dims, extents, g) || !u || !eps) {
+    fprintf(stderr, "ls_ws_symgrad: invalid arguments\n");
+    return -1;
+  }
+  k_symgrad<<<dim3(nblocks(g.nper), g.S), kThreads, 0, (hipStream_t)stream>>>(g, u, add_unit ? 1 : 0, eps);
+  return launch_ok("symgrad");
+}
+
+int ls_ws_symgrad_backward(const int* dims, const float* extents, const uint16_t* grad_eps, int ldg,
+                           uint16_t* grad_u, int ldu, void* stream) {
+  RlGeo g;
+  if (!rl_geo(dims, extents, g) || !grad_eps || !grad_u || ldg < 36 || ldu < 18 || ldu % 8) {
+    fprintf(stderr, "ls_ws_symgrad_backward: invalid arguments\n");
+    return -1;
+  }
+  k_symgrad_back<<<dim3(nblocks(g.nper), g.S), kThreads, 0, (hipStream_t)stream>>>(g, grad_eps, ldg, grad_u, ldu);
+  return launch_ok("symgrad_backward");
+}
+
+int ls_ws_residual_loss(const int* dims, const float* extents, const float* pred, const uint16_t* target,
+                        const float* lam, const float* mu, float wm, float wr, uint16_t* grad, int ldg, float* scratch,
+                        long long scratch_cap, double* part, long long part_cap, float* scale, double* loss,
+                        void* stream) {
+  RlGeo g;
+  if (!rl_geo(dims, extents, g) || !pred || !target || !lam || !mu || !grad || !scratch || !part || !scale ||
+      !loss || ldg < 36 || ldg % 8) {
+    fprintf(stderr, "ls_ws_residual_loss: invalid arguments\n");
+    return -1;
+  }
+  const unsigned nb = nblocks(g.nper);
+  if (scratch_cap < (long long)g.S * 54 * g.nper || part_cap < (long long)g.S * nb * 4 * kRlK) {
+    fprintf(stderr, "ls_ws_residual_loss: scratch (%lld floats) or partial buffer (%lld) too small\n", scratch_cap,
+            part_cap);
+    return -1;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  float* sc = scratch;
+  float* dv = scratch + (size_t)g.S * 36 * g.nper;
+  const dim3 grid(nb, g.S);
+  k_rl_stress<<<grid, kThreads, 0, st>>>(g, pred, target, lam, mu, sc, part);
+  k_rl_div<<<grid, kThreads, 0, st>>>(g, sc, lam, mu, dv, part);
+  k_rl_reduce<<<1, kThreads, 0, st>>>(g.S, (int)nb, wm, wr, part, scale, loss);
+  k_rl_grad<<<grid, kThreads, 0, st>>>(g, sc, lam, mu, dv, scale, grad, ldg);
+  return launch_ok("residual_loss");
 }
 
 int ls_ws_adam(float* w, const float* g, float* m, float* v, long long n, float lr, float b1, float b2, float eps,

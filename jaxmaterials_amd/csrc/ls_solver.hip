@@ -1136,12 +1136,15 @@ int solve_impl(ls_solver* s, const float* d_lambda, const float* d_mu, const int
       }
       if (put_state(s)) return -1;
     }
-    // 2. project the start onto compatible fields with mean eps_bar (FP32)
-    zline<Z_PROJ>(s, d_lambda, d_mu, d_eps);
-    ypass(s, false);
-    xpass(s, 1, false);
-    ypass(s, true);
-    zline<Z_SET>(s, d_lambda, d_mu, d_eps);
+    // 2. project the start onto compatible fields with mean eps_bar (FP32),
+    //    unless the caller's start is compatible already (LS_COMPATIBLE_START)
+    if (!(flags & LS_COMPATIBLE_START)) {
+      zline<Z_PROJ>(s, d_lambda, d_mu, d_eps);
+      ypass(s, false);
+      xpass(s, 1, false);
+      ypass(s, true);
+      zline<Z_SET>(s, d_lambda, d_mu, d_eps);
+    }
     // 3. base = projected start, correction 0; evaluate
     init_fields(s, nullptr, d_eps, d_eps);
     base_eval_active(s, d_lambda, d_mu, d_tau);

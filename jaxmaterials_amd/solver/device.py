@@ -71,7 +71,7 @@ class LSSolver:
 
     def solve(self, lmbda, mu, epsilon_bar, rhs_mat=None, tau=None, epsilon=None, warm_start=False,
               want_sigma=False, rtol=1e-6, atol=0.0, maxiter=1000, rel_ref=0.0, absolute_residual=False,
-              verbose=0, check=True, graph=False, cg=False):
+              verbose=0, check=True, graph=False, cg=False, compatible_start=False):
         """Solve all load cases.
 
         :arg lmbda, mu: GPU float32 [nmat, nx, ny, nz] (or [nx, ny, nz] when nmat == 1)
@@ -84,6 +84,8 @@ class LSSolver:
         :arg graph: replay the iterations between host synchronisations as a captured HIP graph
         :arg cg: conjugate gradients instead of the Lippmann-Schwinger / Anderson update (same
             equation, residual, stop test and iteration semantics; Hooke's law only)
+        :arg compatible_start: the warm start is compatible with mean epsilon_bar already
+            (its projection is skipped)
         Returns a dict with epsilon, sigma (or None), iters, residual, sigma_mean (float64 [nrhs, 6]).
         """
         R = self.nrhs
@@ -109,7 +111,8 @@ class LSSolver:
         rc = self.lib.ls_solver_solve(
             self._h, _ptr(lm), _ptr(m), mats.ctypes.data_as(ctypes.c_void_p), eb.ctypes.data_as(ctypes.c_void_p),
             _ptr(tau), _ptr(epsilon), _ptr(sigma), int(bool(warm_start)), float(rtol), float(atol), int(maxiter),
-            float(rel_ref), (1 if absolute_residual else 0) | (2 if graph else 0) | (4 if cg else 0), int(verbose),
+            float(rel_ref), (1 if absolute_residual else 0) | (2 if graph else 0) | (4 if cg else 0)
+            | (8 if compatible_start else 0), int(verbose),
             iters.ctypes.data_as(ctypes.c_void_p),
             res.ctypes.data_as(ctypes.c_void_p), smean.ctypes.data_as(ctypes.c_void_p),
         )

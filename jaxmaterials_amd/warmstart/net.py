@@ -9,9 +9,15 @@ src/jaxmaterials/solver/lippmann_schwinger.py:181-183).
 Architecture: levels of 16, 32, 64, 128 channels, two 3x3x3 periodic
 convolutions per level with leaky ReLU (slope 0.1), 2x average pooling down,
 nearest 2x upsampling up, decoder inputs [upsampled, skip]; input
-(log lambda - a0) s0, (log mu - a1) s1 and the six components of eps_bar;
-output the six-component strain fluctuation.  1 464 480 weights + 710
-biases (the head is stored with 8 output rows, 2 of them zero).
+(log lambda - a0) s0, (log mu - a1) s1 (six more input channels stay zero);
+output the displacement fluctuations of all six unit load cases at once, 18
+channels (channel 3 k + c = component c under eps_bar = e_k): the
+fluctuation is linear in eps_bar, so one evaluation per microstructure
+serves its whole effective-tensor solve.  The strain is the discrete
+symmetric gradient sym(D^+ u) of the rotated staggered grid
+(ls_ws_symgrad), compatible by construction, 36 channels (6 k + c).
+1 469 664 weights + 722 biases (the head is stored with 24 output rows, 6 of
+them zero).
 """
 
 import ctypes
@@ -33,9 +39,10 @@ LAYERS = (
     ("dec3a", 192, 64, True), ("dec3b", 64, 64, True),
     ("dec2a", 96, 32, True), ("dec2b", 32, 32, True),
     ("dec1a", 48, 16, True), ("dec1b", 16, 16, True),
-    ("head", 16, 8, False),
+    ("head", 16, 24, False),
 )
-HEAD_VALID = 6
+HEAD_VALID = 18  # six unit load cases x three displacement components
+NOUT = 36        # six unit load cases x six strain components
 ACT, DACT, PLANAR = 1, 2, 4
 
 # log-space centre and inverse half-range of the cross-ply phase moduli
@@ -113,6 +120,7 @@ class WarmStartNet:
         self.part = torch.empty(1 << 24, dtype=torch.float32, device=self.device)
         self.part64 = torch.empty(1 << 20, dtype=torch.float64, device=self.device)
         self.loss_dev = torch.zeros(1, dtype=torch.float64, device=self.device)
+        self.loss_scale = torch.zeros(1 << 12, dtype=torch.float32, device=self.device)
         self._norm_dev = torch.tensor(self.norm, dtype=torch.float32, device=self.device)
         self._acts = {}
         self.init_params(seed)
@@ -214,25 +222,32 @@ class WarmStartNet:
         _check(getattr(self.lib, fn)(dims_a, *args, self.stream()), fn)
 
     # -- forward ------------------------------------------------------------
-    def forward(self, lam, mu, eps_bar, per_mat, out, add_eps_bar=True):
-        """Predict the strain of B = len(eps_bar) instances on one grid.
+    def forward(self, lam, mu, out, add_eps_bar=True, extents=None):
+        """Predict the strains of the six unit load cases of S microstructures
+        on one grid.
 
-        lam, mu: float32 device tensors [nmat][X][Y][Z] (instance b uses
-        material b // per_mat); eps_bar: float32 device tensor [B][6];
-        out: float32 device tensor [B][6][X][Y][Z], receives eps_bar +
-        fluctuation (or the fluctuation alone if add_eps_bar is False).
+        lam, mu: float32 device tensors [S][X][Y][Z]; out: float32 device tensor
+        [S][36][X][Y][Z] (equivalently [6 S][6][X][Y][Z], the solver's strain
+        layout for the unit load cases of each microstructure), receives
+        e_k + fluctuation (or the fluctuation alone if add_eps_bar is False);
+        extents = (Lx, Ly, Lz) of the box (default: one unit per voxel).
         Keeps the activations for backward()."""
         X, Y, Z = lam.shape[-3:]
         if X % 8 or Y % 8 or Z % 8:
             raise ValueError("grid sizes must be multiples of 8 (three 2x poolings)")
-        B = eps_bar.shape[0]
+        B = lam.reshape((-1, X, Y, Z)).shape[0]
         nper = X * Y * Z
-        if out.shape != (B, 6, X, Y, Z) or out.dtype != torch.float32 or not out.is_contiguous():
-            raise ValueError("out must be a contiguous float32 tensor [B][6][X][Y][Z]")
+        if out.numel() != B * NOUT * nper or out.dtype != torch.float32 or not out.is_contiguous():
+            raise ValueError("out must be a contiguous float32 tensor of [S][36][X][Y][Z] elements")
         lay = self.layers
         d1, d2, d3, d4 = ((B, X >> s, Y >> s, Z >> s) for s in range(4))
         n1, n2, n3, n4 = (d[0] * d[1] * d[2] * d[3] for d in (d1, d2, d3, d4))
         st = self.stream()
+        if getattr(self, "_eb0", None) is None or self._eb0.shape[0] < B:
+            self._eb0 = torch.zeros((B, 6), dtype=torch.float32, device=self.device)
+        eps_bar = self._eb0[:B]
+        self._extents = np.asarray(extents if extents is not None else (X, Y, Z), dtype=np.float32)
+        per_mat = 1
         x0 = self._buf("x0", n1, 8)
         _check(self.lib.ls_ws_input(_p(lam), _p(mu), per_mat, _p(eps_bar), B, nper, _p(self._norm_dev), _p(x0), st),
                "input")
@@ -273,29 +288,48 @@ class WarmStartNet:
         y1 = self._buf("y1", n1, 16)
         self._conv(lay["dec1a"], d1, cat1, 48, u1, 16, ACT)
         self._conv(lay["dec1b"], d1, u1, 16, y1, 16, ACT)
-        self._conv(lay["head"], d1, y1, 16, out, 0, PLANAR, addc=eps_bar if add_eps_bar else None,
-                   cvalid=HEAD_VALID)
+        u = self._buf("u_head", n1, HEAD_VALID, dtype=torch.float32)
+        self._conv(lay["head"], d1, y1, 16, u, 0, PLANAR, cvalid=HEAD_VALID)
+        dims_a = (ctypes.c_int * 4)(B, X, Y, Z)
+        _check(self.lib.ls_ws_symgrad(dims_a, self._extents.ctypes.data_as(ctypes.c_void_p), _p(u),
+                                      1 if add_eps_bar else 0, _p(out), st), "symgrad")
         self._dims = (d1, d2, d3, d4)
         return out
 
     # -- backward -----------------------------------------------------------
-    def loss_backward(self, pred, target):
-        """MSE of pred (float32 [B][6][X][Y][Z], the fluctuation from
-        forward(..., add_eps_bar=False)) against target (int16-stored bf16
-        [B][6][X][Y][Z]); fills self.grads.  Returns the loss (device tensor)."""
+    def loss_backward(self, pred, target, lam, mu, wm=1.0, wr=0.0, backward=True):
+        """Loss of pred (float32 [S][36][X][Y][Z], the strain fluctuation from
+        forward(..., add_eps_bar=False)) against target (int16-stored bf16, same
+        shape) on the solver's own terms (ls_ws_residual_loss: wm times the
+        relative strain energy of the error, the norm conjugate gradients
+        reduce, plus wr times the squared equilibrium residual of the predicted
+        strain relative to the uniform start's), for the Lame fields
+        lam, mu [S][X][Y][Z] on the box of the last forward(); fills self.grads
+        (with backward).  Returns the loss (device tensor)."""
         d1, d2, d3, d4 = self._dims
         n1, n2, n3, n4 = (d[0] * d[1] * d[2] * d[3] for d in (d1, d2, d3, d4))
-        B, nper = d1[0], d1[1] * d1[2] * d1[3]
-        count = B * 6 * nper
+        S, nper = d1[0], d1[1] * d1[2] * d1[3]
         st = self.stream()
         lay = self.layers
         A = self._acts
-        gh = self._buf("g_head", n1, 8)
-        nb = (n1 + 255) // 256
-        if nb > self.part64.numel():
-            self.part64 = torch.empty(nb, dtype=torch.float64, device=self.device)
-        _check(self.lib.ls_ws_mse(_p(pred), _p(target), B, nper, 2.0 / count, _p(gh), _p(self.part64),
-                                  self.part64.numel(), _p(self.loss_dev), st), "mse")
+        hc = lay["head"].cout
+        ge = self._buf("g_eps", n1, 40)
+        scratch = self._buf("rl_scratch", S * nper, 54, dtype=torch.float32)
+        nb = (nper + 255) // 256
+        if 24 * S * nb > self.part64.numel():
+            self.part64 = torch.empty(24 * S * nb, dtype=torch.float64, device=self.device)
+        if 12 * S > self.loss_scale.numel():
+            self.loss_scale = torch.zeros(12 * S, dtype=torch.float32, device=self.device)
+        dims_a = (ctypes.c_int * 4)(S, d1[1], d1[2], d1[3])
+        ext = self._extents.ctypes.data_as(ctypes.c_void_p)
+        _check(self.lib.ls_ws_residual_loss(dims_a, ext, _p(pred), _p(target), _p(lam), _p(mu), float(wm), float(wr),
+                                            _p(ge), 40,
+                                            _p(scratch), scratch.numel(), _p(self.part64), self.part64.numel(),
+                                            _p(self.loss_scale), _p(self.loss_dev), st), "residual loss")
+        if not backward:
+            return self.loss_dev
+        gh = self._buf("g_head", n1, hc)
+        _check(self.lib.ls_ws_symgrad_backward(dims_a, ext, _p(ge), 40, _p(gh), hc, st), "symgrad backward")
         self.grads.zero_()
         # decoder level 1 and head
         self._wgrad(lay["head"], d1, gh, A["y1"], 16)
