@@ -134,6 +134,42 @@ def cpu_baseline_sample(lam, mu, gs, gpu_iters, timed=2):
     }
 
 
+BATCH_METRIC = ("C_eff solves/sec: uneven batch of 384 seeded cross-ply microstructures on grids of "
+                "32^3-128^3, six load cases each, to rtol 1e-6")
+
+
+def batch_record(depth=4, cg=False, count=384):
+    """The uneven family of 384 seeded microstructures (grids uniform over
+    32^3, 48^3, 64^3, 96^3, 128^3; jaxmaterials_amd.data.batch) solved grid
+    group by grid group, concurrently on their own streams, with graph replay
+    and autotuned tiles.  The time runs from the moment every grid has been
+    tuned (generation of the later microstructures included)."""
+    import numpy as np
+
+    from jaxmaterials_amd.data.batch import sample_params, solve_batch
+
+    params = [sample_params(i) for i in range(count)]
+    res = solve_batch(params, depth=depth, graph=True, autotune=True, concurrent=True, cg=cg)
+    its = res["iters"]
+    return {"microstructures": count, "method": "cg" if cg else f"aa{depth}", "wall_s": res["wall_s"],
+            "ceff_solves_per_s": count / res["wall_s"], "iterations_mean": float(np.mean(its)),
+            "grids": {str(k): {kk: vv for kk, vv in v.items() if kk != "autotune"} for k, v in res["timing"].items()},
+            "note": "grids <= 128^3: one grid group's working set fits in part of the 256 MB Infinity Cache"}
+
+
+def batch_main(args):
+    recs = [batch_record(depth=4), batch_record(cg=True)]
+    best = max(recs, key=lambda r: r["ceff_solves_per_s"])
+    line = {"metric": BATCH_METRIC, "value": best["ceff_solves_per_s"], "unit": "solves/s", "n_gpus": 1,
+            "steps": 1, "warmup": 0, "ms_per_step": best["wall_s"] * 1e3, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+            "config": {"workload": "uneven batch of 384 seeded cross-ply microstructures, grids 32^3-128^3, layers "
+                                   "2-8, radius 1.5-5, vf 0.3-0.6, mu ratio 5-100, six load cases each, rtol 1e-6",
+                       "method": best["method"]},
+            "methods": {r["method"]: r for r in recs}, "roofline": None, "cpu_baseline": None}
+    print(json.dumps(line))
+
+
 DATASET_METRIC = ("dataset samples/sec: reference layered-fibre generator at 160x160x64 "
                   "(placement + rasterisation + C_eff at rtol = atol = 1e-4)")
 
@@ -375,9 +411,11 @@ def main():
                          "warm start")
     ap.add_argument("--rtol", type=float, default=1e-6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=("crossply256", "dataset", "phasefield"), default="crossply256",
-                    help="crossply256: the C_eff solve benchmark (default); dataset: the reference's "
-                         "layered-fibre dataset generation; phasefield: the reference's phase-field script")
+    ap.add_argument("--workload", choices=("crossply256", "batch384", "dataset", "phasefield"), default="crossply256",
+                    help="crossply256: the C_eff solve benchmark (default); batch384: the uneven batch of 384 "
+                         "microstructures; dataset: the reference's layered-fibre dataset generation; phasefield: "
+                         "the reference's phase-field script")
+    ap.add_argument("--no-batch", action="store_true", help="crossply256: leave out the 384-microstructure batch")
     ap.add_argument("--samples", type=int, default=128, help="dataset workload: samples per step")
     ap.add_argument("--batch", type=int, default=32, help="dataset workload: samples per solver call")
     args = ap.parse_args()
@@ -391,8 +429,10 @@ def main():
     if args.workload == "phasefield":
         phasefield_main(args)
         return
+    if args.workload == "batch384":
+        batch_main(args)
+        return
 
-    os.environ["LS_TIMING"] = "1"
     import numpy as np
     import torch
 
@@ -440,40 +480,50 @@ def main():
             from jaxmaterials_amd.warmstart.train import DEFAULT_WEIGHTS, warm_start_solve
             if net is None:
                 net = WarmStartNet.load(DEFAULT_WEIGHTS)
-            r_init = warm_start_solve(net, solver, lam, mu, loads, epsilon=eps, rtol=args.rtol, maxiter=0,
-                                      check=False, cg=use_cg)["residual"]
+            st0 = warm_start_solve(net, solver, lam, mu, loads, epsilon=eps, rtol=args.rtol, maxiter=0,
+                                   check=False, cg=use_cg)
+            r_init = st0["residual"]
+            # equilibrium residual norm relative to the uniform start's (R divides by each start's mean stress)
+            la, mm = float(lam.double().mean()), float(mu.double().mean())
+            m0 = np.array([[la + 2 * mm * (c == k) if c < 3 and k < 3 else (2 * mm if c == k else 0.0)
+                            for c in range(6)] for k in range(6)])
+            wv = np.array([1, 1, 1, 2, 2, 2.0])
+            a_init = r_init * np.sqrt((wv * st0["sigma_mean"] ** 2).sum(1)) / np.sqrt((wv * m0 ** 2).sum(1))
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             net.forward(lam.reshape((1,) + shape), mu.reshape((1,) + shape), eps, extents=ext)
             torch.cuda.synchronize()
             ws_info = {"initial_residual_vs_uniform": [float(v) for v in r_init],
+                       "initial_equilibrium_residual_norm_vs_uniform": [float(v) for v in a_init],
                        "inference_ms": 1e3 * (time.perf_counter() - t0),
                        "weights": os.path.basename(DEFAULT_WEIGHTS)}
 
-        def run():
+        def run(solver=solver, eps=eps, warm=warm, use_cg=use_cg):
+            # the iterations between host synchronisations replayed as captured HIP graphs
             if warm:
                 return warm_start_solve(net, solver, lam, mu, loads, epsilon=eps, rtol=args.rtol, maxiter=3000,
-                                        cg=use_cg)
-            return solver.solve(lam, mu, loads, epsilon=eps, rtol=args.rtol, maxiter=3000, cg=use_cg)
+                                        cg=use_cg, graph=True)
+            return solver.solve(lam, mu, loads, epsilon=eps, rtol=args.rtol, maxiter=3000, cg=use_cg, graph=True)
+
+        def timed(k, run=run):
+            ts, o = [], None
+            for _ in range(k):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                o = run()
+                torch.cuda.synchronize()
+                ts.append(time.perf_counter() - t0)
+            return ts, o
 
         for _ in range(args.warmup):
             run()
-        torch.cuda.synchronize()
-        times = []
-        ms_kind = {k: 0.0 for k in LSSolver.KINDS}
-        cnt_kind = {k: 0 for k in LSSolver.KINDS}
-        out = None
-        nreb = 0
-        for _ in range(args.steps):
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            out = run()
-            torch.cuda.synchronize()
-            times.append(time.perf_counter() - t0)
-            ms, cnt, nreb = solver.last_timing()
-            for k in LSSolver.KINDS:
-                ms_kind[k] += ms[k]
-                cnt_kind[k] += cnt[k]
+        # every method is timed for up to 5 solves; the fastest again for --steps below
+        times, out = timed(min(args.steps, 5))
+        # per-pass kernel times from one more solve, launch by launch with HIP events on the solver stream
+        os.environ["LS_TIMING"] = "1"
+        run()
+        del os.environ["LS_TIMING"]
+        ms_kind, cnt_kind, nreb = solver.last_timing()
         iters = [int(i) for i in out["iters"]]
         C = out["sigma_mean"]
         sec = float(np.mean(times))
@@ -482,16 +532,16 @@ def main():
             if cnt_kind[name] == 0:
                 continue
             b = model_bytes(name, N, M, iters, depth, nreb, use_cg)
-            t = ms_kind[name] / 1e3 / args.steps
-            kinds[name] = {"kernel": KERNEL_OF[name], "ms_total": t * 1e3,
-                           "launches": int(cnt_kind[name] // args.steps),
-                           "avg_launch_ms": t * 1e3 / max(1, cnt_kind[name] // args.steps),
+            t = ms_kind[name] / 1e3
+            kinds[name] = {"kernel": KERNEL_OF[name], "ms_total": t * 1e3, "launches": int(cnt_kind[name]),
+                           "avg_launch_ms": t * 1e3 / max(1, cnt_kind[name]),
                            "algorithmic_bytes": b, "achieved_GBps": (b / t / 1e9) if t > 0 else None}
         loop_bytes = sum(model_bytes(k, N, M, iters, depth, nreb, use_cg) for k in kinds)
-        rec = {"solves_per_s": 1.0 / sec, "ms_per_solve": sec * 1e3, "iters": iters, "depth": depth,
-               "accelerator": "cg" if use_cg else ("anderson" if depth > 0 else "none"),
+        rec = {"solves_per_s": 1.0 / sec, "ms_per_solve": sec * 1e3, "timed_solves": len(times), "iters": iters,
+               "depth": depth, "accelerator": "cg" if use_cg else ("anderson" if depth > 0 else "none"),
                "base_reevaluations": nreb, "passes": kinds, "loop_GBps": loop_bytes / sec / 1e9,
-               "ceff": C.tolist()}
+               "kernel_ms_per_solve": sum(ms_kind.values()), "ceff": C.tolist(), "_run": run, "_timed": timed,
+               "_loop_bytes": loop_bytes}
         if ws_info is not None:
             rec["warm_start"] = ws_info
         if golden is not None:
@@ -501,14 +551,24 @@ def main():
         print(f"# {method}: {rec['ms_per_solve']:.1f} ms/solve, iters {iters}, loop {rec['loop_GBps']:.0f} GB/s, "
               + ", ".join(f"{k} {v['achieved_GBps']:.0f} GB/s ({v['ms_total']:.0f} ms)" for k, v in kinds.items()
                           if v['achieved_GBps']), file=sys.stderr, flush=True)
-        solver.close()
+        rec["_solver"], rec["_eps"] = solver, eps
         del solver, eps
-        if net is not None:
-            net.release()
-        torch.cuda.empty_cache()
 
     best_name = max(results, key=lambda k: results[k]["solves_per_s"])
     best = results[best_name]
+    # the fastest method timed for --steps solves (the headline value)
+    if args.steps > best["timed_solves"]:
+        times, _ = best["_timed"](args.steps)
+        sec = float(np.mean(times))
+        best.update(solves_per_s=1.0 / sec, ms_per_solve=sec * 1e3, timed_solves=len(times),
+                    loop_GBps=best["_loop_bytes"] / sec / 1e9)
+    for k in list(results):
+        results[k]["_solver"].close()
+        for key in [q for q in results[k] if q.startswith("_")]:
+            del results[k][key]
+    if net is not None:
+        net.release()
+    torch.cuda.empty_cache()
     dom = max(best["passes"], key=lambda k: best["passes"][k]["ms_total"])
     dk = best["passes"][dom]
     peak = 8000.0
@@ -553,6 +613,8 @@ def main():
         "cpu_baseline": cpu,
         "methods": {k: {kk: vv for kk, vv in v.items() if kk != "ceff"} for k, v in results.items()},
     }
+    if not args.no_batch and n == 256:
+        line["batch384"] = {"aa4": batch_record(depth=4), "cg": batch_record(cg=True)}
     if "plain" in results and len(results) > 1:
         line["speedup_vs_plain"] = {k: v["solves_per_s"] / results["plain"]["solves_per_s"]
                                     for k, v in results.items() if k != "plain"}

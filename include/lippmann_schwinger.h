@@ -59,7 +59,8 @@ ls_solver *ls_solver_create(const int *voxels, const float *extents, int nrhs,
  * d_tau [nrhs][6][N] or NULL (polarisation added to the stress);
  * d_eps [nrhs][6][N] in/out (input = warm start when warm_start != 0, it is
  * projected onto compatible fields with mean eps_bar and replaced by the
- * uniform start if its residual exceeds the uniform start's); d_sigma
+ * uniform start if its equilibrium residual norm ||xi . sigma_hat|| exceeds
+ * the uniform start's); d_sigma
  * [nrhs][6][N] or NULL.  Host arrays: rhs_mat[nrhs] (material of each load
  * case), eps_bar[nrhs][6]; outputs iters[nrhs], residual[nrhs] (final
  * relative residual), sigma_mean[nrhs][6] (FP64 volume averages), each may

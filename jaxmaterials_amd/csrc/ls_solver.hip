@@ -1120,7 +1120,9 @@ int solve_impl(ls_solver* s, const float* d_lambda, const float* d_mu, const int
 
   // ---- start ----
   if (warm_start) {
-    // 1. residual of the uniform start, the normaliser of the tolerance
+    // 1. residual of the uniform start, the normaliser of the tolerance, and
+    //    its equilibrium residual norm ||xi . sigma_hat|| = R ||sigma_hat(0)||
+    std::vector<double> abs0(s->nrhs, -1.0);
     if (!(rel_ref > 0.0)) {
       init_fields(s, nullptr, nullptr, nullptr);
       base_eval_start(s, d_lambda, d_mu, d_tau);
@@ -1131,6 +1133,7 @@ int solve_impl(ls_solver* s, const float* d_lambda, const float* d_mu, const int
       for (int r = 0; r < s->nrhs; ++r) {
         RhsState& q = s->h_st[r];
         q.R0 = q.R;
+        abs0[r] = s->absolute ? q.R : q.R * q.den;
         q.checks = 0;
         q.active = q.status == 0;
       }
@@ -1151,12 +1154,16 @@ int solve_impl(ls_solver* s, const float* d_lambda, const float* d_mu, const int
     if (zero_spec(s, all)) return -1;
     eval_x(s, drtol, datol, 0, true);
     if (sync_state(s)) return -1;
-    // 4. fall back to the uniform start where the warm start is worse
+    // 4. fall back to the uniform start where the warm start is worse: by the
+    //    equilibrium residual norms of the two starts (their normalised R
+    //    divide by different mean stresses, which differ most for shear load
+    //    cases), or by R against the caller's normaliser rel_ref
     std::vector<int> restart(s->nrhs), resume(s->nrhs);
     bool any_restart = false;
     for (int r = 0; r < s->nrhs; ++r) {
       const RhsState& q = s->h_st[r];
-      restart[r] = (q.status != 0) || (q.R > q.R0);
+      const bool worse = abs0[r] >= 0.0 ? (s->absolute ? q.R : q.R * q.den) > abs0[r] : q.R > q.R0;
+      restart[r] = (q.status != 0) || worse;
       resume[r] = !restart[r] && q.active;
       any_restart = any_restart || restart[r];
     }

@@ -196,12 +196,31 @@ def test_invalid_arguments_are_rejected(gpu):
     assert rc == -1
 
 
+def _norm_v(m):
+    """Voigt norm of each row of mean stresses [R][6] (shear counted twice)."""
+    m = np.asarray(m, np.float64)
+    return np.sqrt(np.sum(np.array([1, 1, 1, 2, 2, 2.0]) * m ** 2, axis=-1))
+
+
+def _uniform_mean_stress(lam, mu):
+    """<C e_k> for the six unit load cases (rows)."""
+    la, m = float(lam.double().mean()), float(mu.double().mean())
+    out = np.zeros((6, 6))
+    for k in range(6):
+        if k < 3:
+            out[k, :3] = la
+            out[k, k] += 2 * m
+        else:
+            out[k, k] = 2 * m
+    return out
+
+
 @pytest.mark.parametrize("n", [64, 128])
 def test_trained_warm_start_vs_uniform_start(gpu, n):
     """What the trained network (jaxmaterials_amd/warmstart/crossply64.npz,
     trained at 64^3) gains on a held-out microstructure of its family at 64^3
-    and at 128^3, for conjugate gradients and Anderson depth 6: the residual
-    of its start relative to the uniform start's (normal load cases below
+    and at 128^3, for conjugate gradients and Anderson depth 6: the equilibrium residual
+    norm of its start relative to the uniform start's (normal load cases below
     WS_MAX_RATIO, none above 1: the solver falls back), fewer iterations in
     total than from the uniform start with CG (Anderson: at most as many,
     within 3), and the same C_eff."""
@@ -221,7 +240,10 @@ def test_trained_warm_start_vs_uniform_start(gpu, n):
     for cg, depth in ((True, 0), (False, 6)):
         solver = LSSolver(gs, nrhs=6, depth=depth)
         ref = solver.solve(lam, mu, loads, rtol=1e-6, maxiter=3000, cg=cg)
-        r0 = warm_start_solve(net, solver, lam, mu, loads, rtol=1e-6, maxiter=0, check=False, cg=cg)["residual"]
+        start = warm_start_solve(net, solver, lam, mu, loads, rtol=1e-6, maxiter=0, check=False, cg=cg)
+        # equilibrium residual norm of the kept start relative to the uniform start's:
+        # R (normalised by the start's own mean stress) times the ratio of the mean stresses
+        r0 = start["residual"] * _norm_v(start["sigma_mean"]) / _norm_v(_uniform_mean_stress(lam, mu))
         out = warm_start_solve(net, solver, lam, mu, loads, rtol=1e-6, maxiter=3000, cg=cg)
         C0, Cw = ref["sigma_mean"], out["sigma_mean"]
         sig = np.abs(C0) > 1e-3 * np.abs(C0).max()
@@ -229,7 +251,7 @@ def test_trained_warm_start_vs_uniform_start(gpu, n):
         print(f"{n}^3 {'CG' if cg else 'AA(6)'}: initial residual vs uniform {np.round(r0, 3).tolist()}; iterations "
               f"uniform {ref['iters'].tolist()} warm {out['iters'].tolist()}; C_eff max rel diff {dC:.2e}", flush=True)
         # the kept start is never worse than the uniform one (the solver falls back)
-        assert np.all(r0 <= 1.0 + 1e-12), r0
+        assert np.all(r0 <= 1.0 + 1e-5), r0
         # the normal load cases start well below the uniform start's residual
         assert np.all(r0[:3] < WS_MAX_RATIO), r0
         if cg:
